@@ -1,0 +1,171 @@
+"""Benchmark: ms per timestep of the FP64 512^3 NS+ACDI MHIT36 step on one MI355X.
+
+Workload (the 512^3 NS+ACDI box): random-phase divergence-free velocity,
+E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (seed 1234); ABC forcing A=B=C=1 at
+k0=2; nu=0.001, rho=1; 512 non-overlapping drops of radius 0.1 (seed 4321),
+eps = dx, gamma = 1.2 umax, sigma = 0.1; dt = 0.1 dx / umax0 (fixed; the
+explicit phase-field update is unstable above CFL ~0.14).  Inputs
+are resident in HBM before the timed region.  One step = every kernel of
+multi/main.f90:329-946 (phase field, momentum + CSF + AB2, divergence, FFT
+Poisson solve, projection, mean removal, umax/CFL).
+
+Prints ONE JSON line.  --gpus above 1 is refused: this build targets one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_step(n, phiflag):
+    """Algorithmic HBM bytes of one step: 39 F (NS+ACDI) or 33 F (NS), F = 8 N^3."""
+    F = 8.0 * n ** 3
+    return (39.0 if phiflag else 33.0) * F
+
+
+def make_inputs(n, phiflag, seed_u=1234, seed_phi=4321, ndrops=512, radius=0.1):
+    from mhit36_amd import ic
+    u, v, w = ic.random_velocity(n, seed=seed_u, k_peak=4.0, urms=1.0)
+    _, dx = ic.grid(n)
+    phi = None
+    if phiflag:
+        eps = dx
+        centres = ic.drop_centres(ndrops, radius, eps, seed=seed_phi)
+        phi = ic.drops(n, centres, radius, eps)
+    umax0 = max(float(u.max()), float(v.max()), float(w.max()))
+    # CFL 0.5 for NS only; 0.1 with ACDI, whose explicit-Euler phase-field
+    # diffusion gamma*eps*dt/dx^2 <= 1/6 with gamma = 1.2 umax, eps = dx needs CFL < 0.14
+    cfl = 0.1 if phiflag else 0.5
+    dt = cfl * dx / umax0
+    return u, v, w, phi, dt
+
+
+def params_for(n, phiflag, dt):
+    return dict(nx=n, dt=dt, mu=0.001, rho=1.0, f1=1.0, f2=1.0, f3=1.0, k0=2.0, radius=0.1,
+                sigma=0.1, epsr=1.0, phiflag=phiflag, inflow=1, inphi=1 if phiflag else 0,
+                gamma_factor=1.2)
+
+
+def cpu_baseline(n, phiflag, fields, dt, steps=2):
+    """Oracle (CPU restatement, C + OpenMP) on the same workload: `steps`
+    steps, the last one timed."""
+    from oracle import orc
+    threads = min(16, os.cpu_count() or 1)
+    orc.set_threads(threads)
+    o = orc.Oracle(**params_for(n, phiflag, dt))
+    for f, a in fields.items():
+        o.set_field(f, a)
+    o.step(steps - 1)
+    t0 = time.perf_counter()
+    o.step(1)
+    ms = (time.perf_counter() - t0) * 1e3
+    o.close()
+    return {"value": round(ms, 3), "unit": "ms/timestep", "cores": threads, "kind": "port",
+            "sample": f"oracle/liboracle.so, {n}^3 {'NS+ACDI' if phiflag else 'NS'} workload above, "
+                      f"{steps} steps, the last one timed, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--phiflag", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mhit36_amd import Solver
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    n, phiflag = args.n, args.phiflag
+    t_ic = time.perf_counter()
+    u, v, w, phi, dt = make_inputs(n, phiflag)
+    fields = {1: u, 2: v, 3: w}
+    if phiflag:
+        fields[5] = phi
+    print(f"[bench] inputs {n}^3 ready in {time.perf_counter() - t_ic:.1f}s, dt={dt:.4e}",
+          file=sys.stderr, flush=True)
+
+    s = Solver(**params_for(n, phiflag, dt))
+    for f, a in fields.items():
+        s.set_field(f, a)
+    s.step(args.warmup)
+    if not args.no_kernel_timing:
+        s.kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d = s.step(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = s.kernel_stats() if not args.no_kernel_timing else []
+    s.kernel_timing(False)
+    dev_bytes = s.device_bytes()
+    s.close()
+    ms = elapsed * 1e3 / args.steps
+    balg = alg_bytes_per_step(n, phiflag)
+    print(f"[bench] {args.steps} steps: {ms:.3f} ms/step, gumax={d.gumax:.4f} cou={d.cou:.3f}",
+          file=sys.stderr, flush=True)
+
+    roof = None
+    kernels = []
+    if stats:
+        for k in stats:
+            avg = k["avg_ms"]
+            gbs = k["bytes"] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+            kernels.append({"name": k["name"], "avg_ms": round(avg, 4), "launches": k["launches"],
+                            "alg_bytes": k["bytes"], "gbs": round(gbs, 1)})
+        dom = max(stats, key=lambda k: k["total_ms"])
+        avg = dom["avg_ms"]
+        ach = dom["bytes"] / (avg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                "avg_ms": round(avg, 4), "alg_bytes_per_launch": dom["bytes"]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n, phiflag, fields, dt)
+
+    out = {
+        "metric": "ms/timestep (FP64 vs mixed) at 512^3 NS+ACDI; HBM GB/s; p99 ensemble latency",
+        "value": round(ms, 4),
+        "unit": "ms/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 {'NS+ACDI' if phiflag else 'NS-only'} FP64 forced HIT"
+                               + (", 512 drops R=0.1" if phiflag else ""),
+                   "nx": n, "phiflag": phiflag, "precision": "fp64", "dt": dt,
+                   "kernel_timing_events": not args.no_kernel_timing},
+        "hbm_gbs_step": round(balg / (ms * 1e-3) / 1e9, 1),
+        "step_roofline_frac": round(balg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "alg_bytes_per_step": balg,
+        "device_bytes": dev_bytes,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

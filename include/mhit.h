@@ -1,0 +1,160 @@
+/*
+ * mhit.h -- C ABI of the MI355X-native MHIT36 timestep (libmhit.so).
+ *
+ * The reference drives one timestep inline in `program main`
+ * (multi/main.f90:329-977) and reaches native code through cuFFT, cuDecomp and
+ * MPI calls.  This header is the drop-in boundary that replaces that inline
+ * body: a Fortran driver keeps main.f90's loop shape (readinput -> init -> IC
+ * -> do t {step; print; dump}) and calls these entry points through
+ * iso_c_binding (mhit36_amd/fortran/mhit_c.f90).  Plain C types only.
+ *
+ * Ownership: the library owns all device memory (fields, FFT spectrum,
+ * reduction partials).  Host buffers belong to the caller and are copied
+ * synchronously; nothing is retained after a call returns.
+ * Threading: one host thread per context; a context is not thread safe.
+ * Errors: every entry point returns 0 (MHIT_OK) or one of MHIT_ERR_*; the
+ * Fortran shim maps non-zero to `error stop`, like CHECK_CUDECOMP_EXIT
+ * (main.f90:1).
+ *
+ * Field layout (readwrite.f90:21-47): fieldn 1=u 2=v 3=w 4=p 5=phi; interior
+ * N x N x N FP64, Fortran order (x fastest), no halo.
+ */
+#ifndef MHIT_H
+#define MHIT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHIT_ABI_VERSION 1
+
+enum {
+    MHIT_OK = 0,
+    MHIT_ERR_HIP = 1,      /* a HIP runtime call failed                         */
+    MHIT_ERR_OOM = 2,      /* device allocation failed                          */
+    MHIT_ERR_BAD_N = 3,    /* nx must be even, >= 4, and of the form 2^a 3^b    */
+    MHIT_ERR_ARG = 4,      /* bad argument (null pointer, bad fieldn, ...)      */
+    MHIT_ERR_CFL = 5,      /* cou > 7 after a step (main.f90:945 `stop`)        */
+    MHIT_ERR_NAN = 6,      /* NaN found in u, v or w after a step               */
+    MHIT_ERR_IO = 7,       /* input.inp or a field file could not be read/written */
+    MHIT_ERR_NODEV = 8,    /* no usable GPU                                     */
+    MHIT_ERR_UNSUPPORTED = 9 /* option not available in this build            */
+};
+
+enum { MHIT_FP64 = 0, MHIT_MIXED = 1 };                    /* precision       */
+enum { MHIT_K_REFERENCE = 0, MHIT_K_MODIFIED = 1 };        /* wavenumbers     */
+
+/* Run parameters: the 16 input.inp values (readinput.f90:13-31), the grid size
+ * that module.f90:2 fixes at compile time, and the switches the reference
+ * hard-codes.  Layout is part of the ABI. */
+typedef struct mhit_params {
+    int nx;          /* grid points per direction (module.f90:2; ny=nz=nx)        */
+    int restart;     /* input.inp line 1: 0 fresh start, 1 restart               */
+    int tstart;      /* line 2: iteration to start from                           */
+    int tfin;        /* line 3: final time step                                   */
+    int dump;        /* line 4: saving frequency                                  */
+    int inflow;      /* line 5: 0 Taylor-Green, 1 read input/{u,v,w}.dat          */
+    int inphi;       /* line 6: 0 spherical drop, 1 read input/phi.dat            */
+    int phiflag;     /* 1: ACDI phase field + CSF on (main.f90:39)                */
+    double dt, mu, rho;           /* lines 7-9                                    */
+    double f1, f2, f3, k0;        /* lines 10-13: ABC forcing                     */
+    double radius, sigma, epsr;   /* lines 14-16: drop radius, sigma, eps/dx      */
+    double gamma_factor;          /* ACDI gamma = gamma_factor*gumax (1.0 = main.f90:362) */
+    int wavenumbers;              /* MHIT_K_REFERENCE (integer k, main.f90:154-159) or MHIT_K_MODIFIED */
+    int poisson_rhs_fp32;         /* 1: round the Poisson RHS to REAL(4) like cmplx(rhsp,0.0) (main.f90:738) */
+    int precision;                /* MHIT_FP64 (MHIT_MIXED: MHIT_ERR_UNSUPPORTED)  */
+    int reserved;
+} mhit_params;
+
+/* Derived constants, readinput.f90:35-43. */
+typedef struct mhit_consts {
+    double twopi, lx, dx, dxi, ddxi, rhoi, eps, epsi, enum_;
+} mhit_consts;
+
+/* Result of mhit_step: the scalars main.f90:906-949 prints or keeps. */
+typedef struct mhit_diag {
+    int steps_done;  /* steps completed by this call                               */
+    int t;           /* index of the last completed step                           */
+    int status;      /* MHIT_OK, MHIT_ERR_CFL or MHIT_ERR_NAN                      */
+    int reserved;
+    double gumax;    /* signed max of u,v,w after mean removal (main.f90:937-941)  */
+    double cou;      /* gumax*dt*dxi (main.f90:942)                                */
+    double umean, vmean, wmean;   /* means removed in the last step (main.f90:906-913) */
+    double ms;       /* wall time of the call, milliseconds                        */
+} mhit_diag;
+
+/* Per-kernel timing collected while kernel timing is on. */
+typedef struct mhit_kstat {
+    char name[32];
+    int launches;
+    int reserved;
+    double total_ms;
+    double bytes_per_launch;   /* algorithmic HBM bytes of one launch            */
+} mhit_kstat;
+
+typedef struct mhit_ctx mhit_ctx;
+
+/* Fills defaults matching multi/input.inp and main.f90 switches. */
+int mhit_default_params(mhit_params *p);
+
+/* readinput (readinput.f90:11-31): 16 positional lines of `path`; nx is taken
+ * from the argument (compile-time in the reference). */
+int mhit_read_input(const char *path, int nx, mhit_params *p);
+
+/* readinput.f90:35-43 */
+int mhit_derive(const mhit_params *p, mhit_consts *c);
+
+/* Replaces main.f90:46-200: device binding, grid (149-161), wavenumber table,
+ * FFT plans (134-146) and field allocation (175-200).  Fields start at zero;
+ * gumax starts at 1 (main.f90:325); the first step is Euler (323-324). */
+int mhit_create(const mhit_params *p, mhit_ctx **ctx);
+int mhit_destroy(mhit_ctx *ctx);
+
+/* Taylor-Green velocity if inflow == 0 (main.f90:218-233) and the spherical
+ * drop if phiflag && inphi == 0 (main.f90:263-278). */
+int mhit_init_fields(mhit_ctx *ctx);
+
+/* Copy an interior field in (readfield payload, readwrite.f90:141-154) or out
+ * (writefield payload, readwrite.f90:29-40). */
+int mhit_set_field(mhit_ctx *ctx, int fieldn, const double *host);
+int mhit_get_field(mhit_ctx *ctx, int fieldn, double *host);
+
+/* Run `nsteps` timesteps (main.f90:329-946 each).  Returns after the device
+ * work completes.  Stops early, like the reference's `stop`, when cou > 7
+ * (MHIT_ERR_CFL) or when u, v or w holds a NaN (MHIT_ERR_NAN). */
+int mhit_step(mhit_ctx *ctx, int nsteps, mhit_diag *diag);
+
+/* Set the step counter and make the next step an Euler step (restart). */
+int mhit_reset_time(mhit_ctx *ctx, int tstart);
+
+/* writefield (readwrite.f90:1-98): raw FP64, Fortran order, no header, to
+ * <dir>/{u,v,w,p,phi}_%08d.dat. */
+int mhit_write_field(mhit_ctx *ctx, const char *dir, int t, int fieldn);
+/* readfield / readfield_restart (readwrite.f90:107-318): read <path> into a field. */
+int mhit_read_field(mhit_ctx *ctx, const char *path, int fieldn);
+
+/* Kernel timing with HIP events on the context's stream. */
+int mhit_kernel_timing(mhit_ctx *ctx, int enable);
+int mhit_kernel_stats(mhit_ctx *ctx, mhit_kstat *out, int max_entries, int *n_entries);
+
+/* Device memory held by the context, bytes. */
+int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
+
+/* Diagnostics (no counterpart in the reference): evaluate the device's own
+ * elementary functions on `count` arguments (op 0 log, 1 tanh, 2 sqrt,
+ * 3 reciprocal), and copy an intermediate device buffer of the last step
+ * (which: 10 psi, 11-13 sharpening flux x/y/z, 14 rhs phi, 15-17 CSF force
+ * x/y/z, 18-20 u*, v*, w*, 21-23 rhs_o u/v/w).  Used by the parity tests to
+ * localise a difference. */
+int mhit_debug_math(int op, size_t count, const double *x, double *y);
+int mhit_debug_buffer(mhit_ctx *ctx, int which, double *host);
+
+const char *mhit_strerror(int code);
+int mhit_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

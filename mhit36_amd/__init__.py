@@ -1,0 +1,12 @@
+"""MI355X-native MHIT36 timestep: HIP kernels behind the C ABI of include/mhit.h.
+
+Importing the package does not touch the GPU; `Solver` loads libmhit.so and
+creates a device context.
+"""
+from ._lib import (MHIT_OK, MHIT_ERR_CFL, MHIT_ERR_NAN, MHIT_FP64, MHIT_MIXED,
+                   MHIT_K_REFERENCE, MHIT_K_MODIFIED, MhitError, Params)
+from .solver import Solver, default_params, read_input, derive, FIELD_NAMES
+
+__all__ = ["Solver", "default_params", "read_input", "derive", "FIELD_NAMES", "MhitError",
+           "Params", "MHIT_OK", "MHIT_ERR_CFL", "MHIT_ERR_NAN", "MHIT_FP64", "MHIT_MIXED",
+           "MHIT_K_REFERENCE", "MHIT_K_MODIFIED"]

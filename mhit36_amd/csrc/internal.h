@@ -1,0 +1,92 @@
+// internal.h -- declarations shared by the libmhit host code and its kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mhit {
+
+constexpr int MAXSTAGES = 12;
+
+// One 1-D FFT plan: radices, sub-transform lengths Ns (product of the earlier
+// radices) and the offset of each stage's twiddles in the twiddle table.
+struct FftPlan {
+    int n, nst;
+    int R[MAXSTAGES];
+    int Ns[MAXSTAGES];
+    int twoff[MAXSTAGES];
+};
+
+// Device-resident scalars carried from one step to the next.  u, v and w are
+// stored before their mean is removed; every reader subtracts umean/vmean/wmean
+// (the same rounding as storing u - umean).
+struct Scalars {
+    double umean, vmean, wmean;   // means removed at the end of the last step
+    double gumax;                 // signed max of u, v, w after mean removal
+    double cou;
+    int halted;                   // 1 after CFL > 7 or NaN: later kernels do nothing
+    int status;                   // 0, 5 (CFL) or 6 (NaN)
+    int steps;                    // steps completed since the context was made
+    int pad;
+};
+
+// Physical constants needed by the kernels (readinput.f90:35-43 + input.inp).
+struct Consts {
+    int n;
+    int phiflag;
+    int fp32_rhs;
+    int pad;
+    double dt, mu, rho, dxi, ddxi, rhoi, eps, epsi, enumr;
+    double f1, f2, f3, sigma, gamma_factor;
+};
+
+// Device buffers of one context.
+struct Fields {
+    double *u, *v, *w;           // velocity (stored before mean removal)
+    double *us, *vs, *ws;        // provisional velocity u*
+    double *ruo, *rvo, *rwo;     // rhs of the previous step (AB2)
+    double *phi, *phin;          // phase field, current and next
+    double *rphi;                // rhs of phi (convective + diffusive)
+    double *psi;                 // distance function
+    double *sx, *sy, *sz;        // sharpening flux a*n at cell centres
+    double *fx, *fy, *fz;        // CSF force at cell centres
+    double *p;                   // pressure
+    double2 *spec;               // half spectrum, [k][j][kx], leading dim ld
+    double *lsum;                // per x-line partial sums, 3 * n * n
+    double *lmax;                // per x-line maxima, 3 * n * n
+    double *psum;                // per z-plane sums, 3 * n
+    double *pmax;                // per z-plane maxima, 3 * n
+    double *sinf, *cosf;         // sin(k0 x_i), cos(k0 x_i)
+    double *ktab;                // wavenumber table
+    double2 *tw;                 // FFT twiddles
+    Scalars *sc;
+    int ld;                      // spectrum leading dimension (>= n/2+1)
+};
+
+// Launch helpers (kernels.hip).  Each returns the name and the algorithmic
+// bytes of the launch through `info` so that the host can time it.
+struct LaunchInfo {
+    const char *name;
+    double bytes;
+};
+
+void launch_phi_rhs(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_psi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_sflux(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_phi_update(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_csf(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta,
+                     hipStream_t s, LaunchInfo *li);
+void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li);
+void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse,
+                  hipStream_t s, LaunchInfo *li);
+void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s,
+                     LaunchInfo *li);
+void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li);
+void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s);
+
+}  // namespace mhit

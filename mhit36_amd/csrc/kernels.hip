@@ -1,0 +1,736 @@
+// kernels.hip -- HIP kernels of one MHIT36 timestep for gfx950 (CDNA4).
+//
+// Fields are N^3 FP64 arrays, x fastest, periodic in all three directions by
+// index wrap (the y/z halo planes of the reference's X-pencils hold periodic
+// copies at a 1x1 decomposition, so wrap reads the same values).  Every
+// expression is evaluated in the order the reference's Fortran writes it
+// (multi/main.f90), and the library is compiled with -ffp-contract=off.
+#include "internal.h"
+#include "devmath.h"
+
+namespace mhit {
+
+#define IDX(i, j, k) ((size_t)(i) + (size_t)n * ((size_t)(j) + (size_t)n * (size_t)(k)))
+
+constexpr int BLK = 256;
+
+__device__ __forceinline__ bool halted(const Scalars *sc) { return sc->halted != 0; }
+
+static inline unsigned nblocks(size_t work, int blk) { return (unsigned)((work + blk - 1) / blk); }
+
+// ---------------------------------------------------------------------------
+// Phase field, main.f90:338-486
+// ---------------------------------------------------------------------------
+// A4 + A5: convective and diffusive terms of rhs phi (main.f90:341-381).
+__global__ void __launch_bounds__(BLK) k_phi_rhs(const double *__restrict__ u, const double *__restrict__ v,
+                                                 const double *__restrict__ w, const double *__restrict__ phi,
+                                                 double *__restrict__ rphi, const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)n * n * n) return;
+    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
+    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    const double gamma = c.gamma_factor * sc->gumax;
+    const double dxi = c.dxi, ddxi = c.ddxi, eps = c.eps;
+    double f0 = phi[t];
+    double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
+    double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
+    double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
+    double uip = u[IDX(ip, j, k)] - um, u0 = u[t] - um;
+    double vjp = v[IDX(i, jp, k)] - vm, v0 = v[t] - vm;
+    double wkp = w[IDX(i, j, kp)] - wm, w0 = w[t] - wm;
+    double A = uip * 0.5 * (fxp + f0) - u0 * 0.5 * (f0 + fxm);
+    double B = vjp * 0.5 * (fyp + f0) - v0 * 0.5 * (f0 + fym);
+    double C = wkp * 0.5 * (fzp + f0) - w0 * 0.5 * (f0 + fzm);
+    double r = (-(A * dxi) - B * dxi) - C * dxi;
+    double dxx = eps * ((fxp - 2.0 * f0) + fxm) * ddxi;
+    double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;
+    double dzz = eps * ((fzp - 2.0 * f0) + fzm) * ddxi;
+    rphi[t] = r + gamma * ((dxx + dyy) + dzz);
+}
+
+// A6: psi = eps*log((min(phi,1)+enum)/(1-min(phi,1)+enum)) (main.f90:384-393).
+__global__ void __launch_bounds__(BLK) k_psi(const double *__restrict__ phi, double *__restrict__ psi,
+                                             const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)c.n * c.n * c.n) return;
+    double f = phi[t];
+    double val = f < 1.0 ? f : 1.0;
+    psi[t] = c.eps * dlog((val + c.enumr) / ((1.0 - val) + c.enumr));
+}
+
+// A7 + A8 and the per-cell factor of A9: normalised undivided normals times
+// a = 0.25*(1 - tanh^2(0.5*psi*epsi)) (main.f90:397-465).
+__global__ void __launch_bounds__(BLK) k_sflux(const double *__restrict__ psi, double *__restrict__ sx,
+                                               double *__restrict__ sy, double *__restrict__ sz,
+                                               const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)n * n * n) return;
+    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
+    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double tiny = (double)1.0E-16f;
+    double gx = psi[IDX(ip, j, k)] - psi[IDX(im, j, k)];
+    double gy = psi[IDX(i, jp, k)] - psi[IDX(i, jm, k)];
+    double gz = psi[IDX(i, j, kp)] - psi[IDX(i, j, km)];
+    double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+    double th = dtanh((0.5 * psi[t]) * c.epsi);
+    double a = 0.25 * (1.0 - th * th);
+    sx[t] = a * (gx * normod);
+    sy[t] = a * (gy * normod);
+    sz[t] = a * (gz * normod);
+}
+
+// A9 + A10: sharpening flux divergence and Euler update of phi (main.f90:446-480).
+__global__ void __launch_bounds__(BLK) k_phi_update(const double *__restrict__ phi, const double *__restrict__ rphi,
+                                                    const double *__restrict__ sx, const double *__restrict__ sy,
+                                                    const double *__restrict__ sz, double *__restrict__ phin,
+                                                    const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)n * n * n) return;
+    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
+    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double gamma = c.gamma_factor * sc->gumax;
+    double X = (sx[IDX(ip, j, k)] - sx[IDX(im, j, k)]) * 0.5 * c.dxi;
+    double Y = (sy[IDX(i, jp, k)] - sy[IDX(i, jm, k)]) * 0.5 * c.dxi;
+    double Z = (sz[IDX(i, j, kp)] - sz[IDX(i, j, km)]) * 0.5 * c.dxi;
+    double r = rphi[t] - gamma * ((X + Y) + Z);
+    phin[t] = phi[t] + c.dt * r;
+}
+
+// A14, first half: CSF force at cell centres from phi^{n+1} (main.f90:604-627).
+__global__ void __launch_bounds__(BLK) k_csf(const double *__restrict__ phi, double *__restrict__ fx,
+                                             double *__restrict__ fy, double *__restrict__ fz,
+                                             const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)n * n * n) return;
+    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
+    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double sig6 = 6.0 * c.sigma;
+    double f0 = phi[t];
+    double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
+    double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
+    double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
+    double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * c.epsi;
+    double s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - 6.0 * f0;
+    double chem = t1 - c.eps * s6 * c.ddxi;
+    double gx = 0.5 * (fxp - fxm) * c.dxi;
+    double gy = 0.5 * (fyp - fym) * c.dxi;
+    double gz = 0.5 * (fzp - fzm) * c.dxi;
+    fx[t] = sig6 * chem * gx;
+    fy[t] = sig6 * chem * gy;
+    fz[t] = sig6 * chem * gz;
+}
+
+// ---------------------------------------------------------------------------
+// Momentum, main.f90:513-681: A11 convection, A12 viscous, A13 ABC forcing,
+// A14 CSF at the faces, A15 AB2/Euler advance and rhs_o <- rhs.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_momentum(
+    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+    const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fz,
+    double *__restrict__ us, double *__restrict__ vs, double *__restrict__ ws,
+    double *__restrict__ ruo, double *__restrict__ rvo, double *__restrict__ rwo,
+    const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
+    Consts c, double alpha, double beta)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= (size_t)n * n * n) return;
+    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
+    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    const double dxi = c.dxi, ddxi = c.ddxi, mu = c.mu, rhoi = c.rhoi;
+#define U(a, b, cc) (u[IDX(a, b, cc)] - um)
+#define V(a, b, cc) (v[IDX(a, b, cc)] - vm)
+#define W(a, b, cc) (w[IDX(a, b, cc)] - wm)
+    const double u0 = U(i, j, k), v0 = V(i, j, k), w0 = W(i, j, k);
+    const double uxp = U(ip, j, k), uxm = U(im, j, k), uyp = U(i, jp, k), uym = U(i, jm, k);
+    const double uzp = U(i, j, kp), uzm = U(i, j, km);
+    const double vxp = V(ip, j, k), vxm = V(im, j, k), vyp = V(i, jp, k), vym = V(i, jm, k);
+    const double vzp = V(i, j, kp), vzm = V(i, j, km);
+    const double wxp = W(ip, j, k), wxm = W(im, j, k), wyp = W(i, jp, k), wym = W(i, jm, k);
+    const double wzp = W(i, j, kp), wzm = W(i, j, km);
+    double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+    double h12 = (uyp + u0) * (vyp + V(im, jp, k)) - (u0 + uym) * (v0 + vxm);
+    double h13 = (uzp + u0) * (wzp + W(im, j, kp)) - (u0 + uzm) * (w0 + wxm);
+    double h21 = (uxp + U(ip, jm, k)) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+    double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+    double h23 = (wzp + W(i, jm, kp)) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+    double h31 = (wxp + w0) * (uxp + U(ip, j, km)) - (w0 + wxm) * (u0 + uzm);
+    double h32 = (vyp + V(i, jp, km)) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+    double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+#undef U
+#undef V
+#undef W
+    h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
+    h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
+    h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
+    double ru = -((h11 + h12) + h13);
+    double rv = -((h21 + h22) + h23);
+    double rw = -((h31 + h32) + h33);
+    h11 = mu * ((uxp - 2.0 * u0) + uxm) * ddxi;
+    h12 = mu * ((uyp - 2.0 * u0) + uym) * ddxi;
+    h13 = mu * ((uzp - 2.0 * u0) + uzm) * ddxi;
+    h21 = mu * ((vxp - 2.0 * v0) + vxm) * ddxi;
+    h22 = mu * ((vyp - 2.0 * v0) + vym) * ddxi;
+    h23 = mu * ((vzp - 2.0 * v0) + vzm) * ddxi;
+    h31 = mu * ((wxp - 2.0 * w0) + wxm) * ddxi;
+    h32 = mu * ((wyp - 2.0 * w0) + wym) * ddxi;
+    h33 = mu * ((wzp - 2.0 * w0) + wzm) * ddxi;
+    ru = ru + ((h11 + h12) + h13) * rhoi;
+    rv = rv + ((h21 + h22) + h23) * rhoi;
+    rw = rw + ((h31 + h32) + h33) * rhoi;
+    ru = (ru + c.f3 * sinf[k]) + c.f2 * cosf[j];
+    rv = (rv + c.f1 * sinf[i]) + c.f3 * cosf[k];
+    rw = (rw + c.f2 * sinf[j]) + c.f1 * cosf[i];
+    if (c.phiflag) {
+        ru = ru + 0.5 * (fx[IDX(im, j, k)] + fx[t]) * rhoi;
+        rv = rv + 0.5 * (fy[IDX(i, jm, k)] + fy[t]) * rhoi;
+        rw = rw + 0.5 * (fz[IDX(i, j, km)] + fz[t]) * rhoi;
+    }
+    const double dt = c.dt;
+    us[t] = u0 + dt * (alpha * ru - beta * ruo[t]);
+    vs[t] = v0 + dt * (alpha * rv - beta * rvo[t]);
+    ws[t] = w0 + dt * (alpha * rw - beta * rwo[t]);
+    ruo[t] = ru;
+    rvo[t] = rv;
+    rwo[t] = rw;
+}
+
+// ---------------------------------------------------------------------------
+// FFT machinery: mixed-radix Stockham autosort in LDS (Govindaraju et al.,
+// SC'08), unnormalised.  A block holds `nlines` lines of length n; element p of
+// line l sits at lds[l*lstride + p*pstride].
+// ---------------------------------------------------------------------------
+constexpr int FFT_THREADS = 256;
+constexpr int MAXB = 4;   // butterflies per thread per stage
+constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block
+
+template <int R, int SG>
+__device__ __forceinline__ void fft_stage(double2 *lds, int n, int nlines, int lstride, int pstride,
+                                          int Ns, const double2 *__restrict__ tw, bool first)
+{
+    const int L = n / R;
+    const int total = nlines * L;
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        int t = threadIdx.x + b * FFT_THREADS;
+        if (t < total) {
+            int line = t / L, j = t - line * L;
+            const double2 *base = lds + line * lstride;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = base[(j + r * L) * pstride];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        int t = threadIdx.x + b * FFT_THREADS;
+        if (t < total) {
+            int line = t / L, j = t - line * L;
+            int k = j % Ns;
+            if (!first) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 wv = tw[k * (R - 1) + r - 1];
+                    double wi = SG < 0 ? -wv.y : wv.y;
+                    double xr = v[b][r].x, xi = v[b][r].y;
+                    v[b][r] = make_double2(xr * wv.x - xi * wi, xr * wi + xi * wv.x);
+                }
+            }
+            dft<R, SG>(v[b]);
+            int base_idx = (j / Ns) * Ns * R + k;
+            double2 *base = lds + line * lstride;
+#pragma unroll
+            for (int r = 0; r < R; r++) base[(base_idx + r * Ns) * pstride] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int SG>
+__device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, const FftPlan &pl,
+                        const double2 *__restrict__ tw)
+{
+    for (int s = 0; s < pl.nst; s++) {
+        const double2 *t = tw + pl.twoff[s];
+        bool first = (s == 0);
+        switch (pl.R[s]) {
+        case 8: fft_stage<8, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
+        case 4: fft_stage<4, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
+        case 3: fft_stage<3, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
+        default: fft_stage<2, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
+        }
+    }
+}
+
+// A16 + A17 + forward x pass: divergence of u*, REAL(4) rounding of the
+// Poisson RHS, then one complex FFT per pair of x-lines (j, j+1) whose
+// spectrum is split into the two half spectra (main.f90:715-750).
+__global__ void __launch_bounds__(FFT_THREADS) k_xfwd(
+    const double *__restrict__ us, const double *__restrict__ vs, const double *__restrict__ ws,
+    double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
+    Consts c, FftPlan pl, int ppb, int ld)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = c.n, half = n / 2, nh = n / 2 + 1;
+    const int lstride = n + 1;
+    const int pair0 = blockIdx.x * ppb;
+    const double cdiv = c.rho * c.dxi / c.dt;
+    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+        int pl_ = e / n, i = e - pl_ * n;
+        int pair = pair0 + pl_;
+        int k = pair / half, j = 2 * (pair - k * half);
+        int ip = i + 1 == n ? 0 : i + 1;
+        int kp = k + 1 == n ? 0 : k + 1;
+        double rr[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int jj = j + h;
+            int jp = jj + 1 == n ? 0 : jj + 1;
+            size_t id = IDX(i, jj, k);
+            double r = cdiv * (us[IDX(ip, jj, k)] - us[id]);
+            r = r + cdiv * (vs[IDX(i, jp, k)] - vs[id]);
+            r = r + cdiv * (ws[IDX(i, jj, kp)] - ws[id]);
+            if (c.fp32_rhs) r = (double)(float)r;
+            rr[h] = r;
+        }
+        lds[pl_ * lstride + i] = make_double2(rr[0], rr[1]);
+    }
+    __syncthreads();
+    fft_lds<-1>(lds, ppb, lstride, 1, pl, tw);
+    for (int e = threadIdx.x; e < ppb * nh; e += FFT_THREADS) {
+        int pl_ = e / nh, q = e - pl_ * nh;
+        int pair = pair0 + pl_;
+        int k = pair / half, j = 2 * (pair - k * half);
+        int m = q == 0 ? 0 : n - q;
+        double2 zq = lds[pl_ * lstride + q], zm = lds[pl_ * lstride + m];
+        double a = zq.x, b = zq.y, cc = zm.x, d = zm.y;
+        spec[((size_t)k * n + j) * ld + q] = make_double2((a + cc) * 0.5, (b - d) * 0.5);
+        spec[((size_t)k * n + j + 1) * ld + q] = make_double2((b + d) * 0.5, (cc - a) * 0.5);
+    }
+}
+
+// Column pass along y (forward or inverse), in place.  Block = (kx tile, k).
+template <int SG>
+__global__ void __launch_bounds__(FFT_THREADS) k_ypass(double2 *__restrict__ spec, const double2 *__restrict__ tw,
+                                                       const Scalars *__restrict__ sc, FftPlan pl, int ld, int tkx)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, nh = n / 2 + 1;
+    const int q0 = blockIdx.x * tkx, k = blockIdx.y;
+    const int nq = min(tkx, nh - q0);
+    double2 *base = spec + (size_t)k * n * ld + q0;
+    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+        int r = e / tkx, ql = e - r * tkx;
+        lds[r * tkx + ql] = ql < nq ? base[(size_t)r * ld + ql] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<SG>(lds, tkx, 1, tkx, pl, tw);
+    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+        int r = e / tkx, ql = e - r * tkx;
+        if (ql < nq) base[(size_t)r * ld + ql] = lds[r * tkx + ql];
+    }
+}
+
+// Column pass along z: forward FFT, division by -k^2 and N^3 with the mean
+// mode set to zero (A19, main.f90:765-798), inverse FFT.  Block = (kx tile, j).
+__global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ spec, const double2 *__restrict__ tw,
+                                                          const double *__restrict__ ktab,
+                                                          const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                          int tkx)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, nh = n / 2 + 1;
+    const int q0 = blockIdx.x * tkx, j = blockIdx.y;
+    const int nq = min(tkx, nh - q0);
+    const size_t rs = (size_t)n * ld;
+    double2 *base = spec + (size_t)j * ld + q0;
+    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+        int r = e / tkx, ql = e - r * tkx;
+        lds[r * tkx + ql] = ql < nq ? base[(size_t)r * rs + ql] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<-1>(lds, tkx, 1, tkx, pl, tw);
+    const double nnn = (double)((long long)n * (long long)n * (long long)n);
+    const double ky = ktab[j];
+    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+        int r = e / tkx, ql = e - r * tkx;
+        int q = q0 + ql;
+        if (ql < nq) {
+            double kx = ktab[q], kz = ktab[r];
+            double k2 = (kx * kx + ky * ky) + kz * kz;
+            double2 z = lds[r * tkx + ql];
+            z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
+            if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
+            lds[r * tkx + ql] = z;
+        }
+    }
+    __syncthreads();
+    fft_lds<1>(lds, tkx, 1, tkx, pl, tw);
+    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+        int r = e / tkx, ql = e - r * tkx;
+        if (ql < nq) base[(size_t)r * rs + ql] = lds[r * tkx + ql];
+    }
+}
+
+// Inverse x pass: Hermitian extension of the half spectra of lines j and j+1,
+// one complex inverse FFT per pair, p(.,j) = Re, p(.,j+1) = Im (A20-A21).
+__global__ void __launch_bounds__(FFT_THREADS) k_xinv(const double2 *__restrict__ spec, double *__restrict__ p,
+                                                      const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
+                                                      FftPlan pl, int ppb, int ld)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, half = n / 2;
+    const int lstride = n + 1;
+    const int pair0 = blockIdx.x * ppb;
+    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+        int pl_ = e / n, q = e - pl_ * n;
+        int pair = pair0 + pl_;
+        int k = pair / half, j = 2 * (pair - k * half);
+        const double2 *s1 = spec + ((size_t)k * n + j) * ld;
+        const double2 *s2 = spec + ((size_t)k * n + j + 1) * ld;
+        double2 z;
+        if (q == 0 || q == half) {
+            z = make_double2(s1[q].x, s2[q].x);
+        } else if (q < half) {
+            double2 a = s1[q], b = s2[q];
+            z = make_double2(a.x - b.y, a.y + b.x);
+        } else {
+            int m = n - q;
+            double2 a = s1[m], b = s2[m];
+            z = make_double2(a.x + b.y, b.x - a.y);
+        }
+        lds[pl_ * lstride + q] = z;
+    }
+    __syncthreads();
+    fft_lds<1>(lds, ppb, lstride, 1, pl, tw);
+    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+        int pl_ = e / n, i = e - pl_ * n;
+        int pair = pair0 + pl_;
+        int k = pair / half, j = 2 * (pair - k * half);
+        double2 z = lds[pl_ * lstride + i];
+        p[IDX(i, j, k)] = z.x;
+        p[IDX(i, j + 1, k)] = z.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A22 projection (main.f90:873-887) with the per-x-line partial sums and
+// maxima behind A23/A25.  One 64-wide wavefront per x-line: lane l owns
+// i = l, l+64, ..., adds them in that order, then a xor-butterfly combines the
+// 64 lanes.  That fixed order is what the oracle's lane-64 sum restates.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nanmax(double a, double b)
+{
+    if (a != a) return a;
+    if (b != b) return b;
+    return a > b ? a : b;
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = x + __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ __forceinline__ double wave_max(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = nanmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+__global__ void __launch_bounds__(BLK) k_project(
+    const double *__restrict__ us, const double *__restrict__ vs, const double *__restrict__ ws,
+    const double *__restrict__ p, double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+    double *__restrict__ lsum, double *__restrict__ lmax, const Scalars *__restrict__ sc, Consts c)
+{
+    if (halted(sc)) return;
+    const int n = c.n;
+    const int lane = threadIdx.x & 63;
+    const size_t line = ((size_t)blockIdx.x * BLK + threadIdx.x) >> 6;
+    if (line >= (size_t)n * n) return;
+    const int j = (int)(line % n), k = (int)(line / n);
+    const int jm = j == 0 ? n - 1 : j - 1, km = k == 0 ? n - 1 : k - 1;
+    const double dtr = c.dt / c.rho, dxi = c.dxi;
+    double su = 0.0, sv = 0.0, sw = 0.0;
+    double mu_ = -__builtin_inf(), mv = -__builtin_inf(), mw = -__builtin_inf();
+    for (int i = lane; i < n; i += 64) {
+        int im = i == 0 ? n - 1 : i - 1;
+        size_t id = IDX(i, j, k);
+        double p0 = p[id];
+        double uu = us[id] - dtr * (p0 - p[IDX(im, j, k)]) * dxi;
+        double vv = vs[id] - dtr * (p0 - p[IDX(i, jm, k)]) * dxi;
+        double ww = ws[id] - dtr * (p0 - p[IDX(i, j, km)]) * dxi;
+        u[id] = uu;
+        v[id] = vv;
+        w[id] = ww;
+        su = su + uu; sv = sv + vv; sw = sw + ww;
+        mu_ = nanmax(mu_, uu); mv = nanmax(mv, vv); mw = nanmax(mw, ww);
+    }
+    su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
+    mu_ = wave_max(mu_); mv = wave_max(mv); mw = wave_max(mw);
+    if (lane == 0) {
+        const size_t nn = (size_t)n * n;
+        lsum[line] = su; lsum[nn + line] = sv; lsum[2 * nn + line] = sw;
+        lmax[line] = mu_; lmax[nn + line] = mv; lmax[2 * nn + line] = mw;
+    }
+}
+
+// Per z-plane: lane-64 sum over the n line sums of the plane (y direction).
+// One wavefront per (plane, component).
+__global__ void __launch_bounds__(64) k_reduce_planes(const double *__restrict__ lsum, const double *__restrict__ lmax,
+                                                      double *__restrict__ psum, double *__restrict__ pmax,
+                                                      const Scalars *__restrict__ sc, int n)
+{
+    if (halted(sc)) return;
+    const int k = blockIdx.x, comp = blockIdx.y, lane = threadIdx.x;
+    const size_t nn = (size_t)n * n;
+    const double *ls = lsum + comp * nn + (size_t)k * n;
+    const double *lm = lmax + comp * nn + (size_t)k * n;
+    double s = 0.0, m = -__builtin_inf();
+    for (int j = lane; j < n; j += 64) {
+        s = s + ls[j];
+        m = nanmax(m, lm[j]);
+    }
+    s = wave_sum(s);
+    m = wave_max(m);
+    if (lane == 0) {
+        psum[comp * n + k] = s;
+        pmax[comp * n + k] = m;
+    }
+}
+
+// Means (main.f90:906-913), signed umax (937-941), CFL (942-945), halt flag.
+__global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum, const double *__restrict__ pmax,
+                                                 Scalars *__restrict__ sc, Consts c)
+{
+    if (sc->halted) return;
+    const int n = c.n, lane = threadIdx.x;
+    double s[3], m[3];
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        double a = 0.0, b = -__builtin_inf();
+        for (int k = lane; k < n; k += 64) {
+            a = a + psum[comp * n + k];
+            b = nanmax(b, pmax[comp * n + k]);
+        }
+        s[comp] = wave_sum(a);
+        m[comp] = wave_max(b);
+    }
+    if (lane == 0) {
+        const double dn = (double)n;
+        double gu = s[0] / dn / dn / dn;
+        double gv = s[1] / dn / dn / dn;
+        double gw = s[2] / dn / dn / dn;
+        // max(fl(x - g)) over x equals fl(max(x) - g): rounding is monotone.
+        double uc = m[0] - gu, vc = m[1] - gv, wc = m[2] - gw;
+        bool nan = (uc != uc) || (vc != vc) || (wc != wc);
+        double mx = uc > vc ? uc : vc;
+        double gumax = wc > mx ? wc : mx;
+        double cou = gumax * c.dt * c.dxi;
+        sc->umean = gu;
+        sc->vmean = gv;
+        sc->wmean = gw;
+        sc->gumax = gumax;
+        sc->cou = cou;
+        sc->steps = sc->steps + 1;
+        if (nan) { sc->halted = 1; sc->status = 6; }
+        else if (cou > 7.0) { sc->halted = 1; sc->status = 5; }
+    }
+}
+
+// Diagnostic: the device's elementary functions on an array of arguments.
+__global__ void k_debug_math(int op, size_t count, const double *__restrict__ x, double *__restrict__ y)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double a = x[t];
+    double r;
+    switch (op) {
+    case 0: r = dlog(a); break;
+    case 1: r = dtanh(a); break;
+    case 2: r = sqrt(a); break;
+    default: r = 1.0 / a; break;
+    }
+    y[t] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s)
+{
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_debug_math, dim3(nblocks(count, BLK)), dim3(BLK), 0, s, op, count, x, y);
+}
+
+static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
+
+void launch_phi_rhs(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_phi_rhs, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.phi, f.rphi, f.sc, c);
+    li->name = "phi_rhs"; li->bytes = 5 * F(c);
+}
+
+void launch_psi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_psi, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.psi, f.sc, c);
+    li->name = "psi"; li->bytes = 2 * F(c);
+}
+
+void launch_sflux(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_sflux, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.psi, f.sx, f.sy, f.sz, f.sc, c);
+    li->name = "sflux"; li->bytes = 4 * F(c);
+}
+
+void launch_phi_update(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_phi_update, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.rphi, f.sx, f.sy, f.sz,
+                       f.phin, f.sc, c);
+    li->name = "phi_update"; li->bytes = 6 * F(c);
+}
+
+void launch_csf(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_csf, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.fx, f.fy, f.fz, f.sc, c);
+    li->name = "csf"; li->bytes = 4 * F(c);
+}
+
+void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta, hipStream_t s,
+                     LaunchInfo *li)
+{
+    size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_momentum, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.fx, f.fy, f.fz, f.us,
+                       f.vs, f.ws, f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta);
+    li->name = "momentum"; li->bytes = (c.phiflag ? 15 : 12) * F(c);
+}
+
+static int pairs_per_block(const FftPlan &pl)
+{
+    // largest power of two p with p*n/Rmin <= MAXB*FFT_THREADS and p <= 8
+    int rmin = 8;
+    for (int s = 0; s < pl.nst; s++) rmin = pl.R[s] < rmin ? pl.R[s] : rmin;
+    int p = 8;
+    while (p > 1 && (p * (pl.n / rmin) > MAXB * FFT_THREADS)) p >>= 1;
+    while (p > 1 && (size_t)p * (pl.n + 1) * sizeof(double2) > LDS_CAP) p >>= 1;
+    int pairs = pl.n * pl.n / 2;
+    while (p > 1 && pairs % p != 0) p >>= 1;
+    return p;
+}
+
+static int col_tile(const FftPlan &pl)
+{
+    int rmin = 8;
+    for (int s = 0; s < pl.nst; s++) rmin = pl.R[s] < rmin ? pl.R[s] : rmin;
+    int t = 8;
+    while (t > 1 && (t * (pl.n / rmin) > MAXB * FFT_THREADS)) t >>= 1;
+    while (t > 1 && (size_t)t * pl.n * sizeof(double2) > LDS_CAP) t >>= 1;
+    return t;
+}
+
+void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+{
+    int ppb = pairs_per_block(pl);
+    int pairs = c.n * c.n / 2;
+    size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
+    hipLaunchKernelGGL(k_xfwd, dim3(pairs / ppb), dim3(FFT_THREADS), shm, s, f.us, f.vs, f.ws, f.spec, f.tw, f.sc,
+                       c, pl, ppb, f.ld);
+    li->name = "xfwd_div"; li->bytes = 4 * F(c);
+}
+
+void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse, hipStream_t s,
+                  LaunchInfo *li)
+{
+    int tkx = col_tile(pl);
+    int nh = c.n / 2 + 1;
+    dim3 grid((nh + tkx - 1) / tkx, c.n);
+    size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+    if (inverse)
+        hipLaunchKernelGGL(k_ypass<1>, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+    else
+        hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+    li->name = inverse ? "y_inv" : "y_fwd";
+    li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
+}
+
+void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+{
+    int tkx = col_tile(pl);
+    int nh = c.n / 2 + 1;
+    dim3 grid((nh + tkx - 1) / tkx, c.n);
+    size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+    hipLaunchKernelGGL(k_zpoisson, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+    li->name = "z_poisson";
+    li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
+}
+
+void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+{
+    int ppb = pairs_per_block(pl);
+    int pairs = c.n * c.n / 2;
+    size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
+    hipLaunchKernelGGL(k_xinv, dim3(pairs / ppb), dim3(FFT_THREADS), shm, s, f.spec, f.p, f.tw, f.sc, pl, ppb,
+                       f.ld);
+    li->name = "x_inv"; li->bytes = 16.0 * (double)(c.n / 2 + 1) * c.n * c.n + F(c);
+}
+
+void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    size_t threads = (size_t)c.n * c.n * 64;
+    hipLaunchKernelGGL(k_project, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, f.us, f.vs, f.ws, f.p, f.u, f.v,
+                       f.w, f.lsum, f.lmax, f.sc, c);
+    li->name = "project"; li->bytes = 7 * F(c);
+}
+
+void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    hipLaunchKernelGGL(k_reduce_planes, dim3(c.n, 3), dim3(64), 0, s, f.lsum, f.lmax, f.psum, f.pmax, f.sc, c.n);
+    li->name = "reduce_planes"; li->bytes = 6.0 * 8.0 * c.n * c.n;
+}
+
+void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, f.psum, f.pmax, f.sc, c);
+    li->name = "finalize"; li->bytes = 6.0 * 8.0 * c.n;
+}
+
+}  // namespace mhit

@@ -1,0 +1,165 @@
+"""Host-side mirror of the reference driver (multi/main.f90) over libmhit.
+
+Arrays follow readwrite.f90's layout: an interior field is N x N x N FP64 in
+Fortran order, exposed here as a numpy array of shape (N, N, N) with
+order='F', so that a[i, j, k] is field(i+1, j+1, k+1) of the reference.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, Consts, Diag, KStat, check, MhitError
+
+FIELD_NAMES = {1: "u", 2: "v", 3: "w", 4: "p", 5: "phi"}   # readwrite.f90:21-25
+
+
+def default_params(**overrides):
+    """mhit_params with multi/input.inp defaults, then `overrides`."""
+    lib = _lib.load()
+    p = Params()
+    check(lib.mhit_default_params(ctypes.byref(p)), "mhit_default_params")
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def read_input(path, nx, **overrides):
+    """readinput (readinput.f90:11-31) of `path`, with nx given at run time."""
+    lib = _lib.load()
+    p = Params()
+    check(lib.mhit_read_input(os.fsencode(path), int(nx), ctypes.byref(p)), f"read_input({path})")
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def derive(params):
+    """Derived constants of readinput.f90:35-43."""
+    lib = _lib.load()
+    c = Consts()
+    check(lib.mhit_derive(ctypes.byref(params), ctypes.byref(c)), "mhit_derive")
+    return {name: getattr(c, name) for name, _ in Consts._fields_}
+
+
+def debug_math(op, x):
+    """Device evaluation of log (0), tanh (1), sqrt (2) or 1/x (3)."""
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    check(lib.mhit_debug_math(int(op), x.size, x.ctypes.data_as(ctypes.c_void_p),
+                              y.ctypes.data_as(ctypes.c_void_p)), "debug_math")
+    return y
+
+
+class Solver:
+    """One periodic box resident on the GPU: fields, FFT plan and state."""
+
+    def __init__(self, params=None, **overrides):
+        lib = _lib.load()
+        if params is None:
+            params = default_params(**overrides)
+        elif isinstance(params, dict):
+            params = default_params(**{**params, **overrides})
+        else:
+            for k, v in overrides.items():
+                setattr(params, k, v)
+        self.params = params
+        self.n = params.nx
+        self._lib = lib
+        h = ctypes.c_void_p()
+        check(lib.mhit_create(ctypes.byref(params), ctypes.byref(h)), "mhit_create")
+        self._h = h
+        self.consts = derive(params)
+        self.last = None
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mhit_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- fields ---------------------------------------------------------------
+    def init_fields(self):
+        """Taylor-Green / spherical drop initial condition (main.f90:218-278)."""
+        check(self._lib.mhit_init_fields(self._h), "mhit_init_fields")
+
+    def set_field(self, fieldn, a):
+        a = np.asarray(a, dtype=np.float64)
+        n = self.n
+        if a.size != n ** 3:
+            raise ValueError(f"field must hold {n}^3 values, got {a.size}")
+        buf = np.asfortranarray(a.reshape((n, n, n), order="F") if a.ndim != 3 else a)
+        check(self._lib.mhit_set_field(self._h, int(fieldn), buf.ctypes.data_as(ctypes.c_void_p)),
+              f"set_field({fieldn})")
+
+    def get_field(self, fieldn):
+        n = self.n
+        out = np.empty((n, n, n), dtype=np.float64, order="F")
+        check(self._lib.mhit_get_field(self._h, int(fieldn), out.ctypes.data_as(ctypes.c_void_p)),
+              f"get_field({fieldn})")
+        return out
+
+    # -- time stepping ----------------------------------------------------------
+    def step(self, nsteps=1, raise_on_stop=True):
+        """Advance `nsteps` timesteps (main.f90:329-946 each)."""
+        d = Diag()
+        rc = self._lib.mhit_step(self._h, int(nsteps), ctypes.byref(d))
+        self.last = d
+        if rc != _lib.MHIT_OK and (raise_on_stop or rc not in (_lib.MHIT_ERR_CFL, _lib.MHIT_ERR_NAN)):
+            raise MhitError(rc, "mhit_step")
+        return d
+
+    def reset_time(self, tstart):
+        check(self._lib.mhit_reset_time(self._h, int(tstart)), "reset_time")
+
+    # -- files (readwrite.f90) ------------------------------------------------
+    def write_field(self, directory, t, fieldn):
+        check(self._lib.mhit_write_field(self._h, os.fsencode(directory), int(t), int(fieldn)),
+              f"write_field({fieldn})")
+
+    def read_field(self, path, fieldn):
+        check(self._lib.mhit_read_field(self._h, os.fsencode(path), int(fieldn)), f"read_field({path})")
+
+    # -- measurement ------------------------------------------------------------
+    def kernel_timing(self, enable=True):
+        check(self._lib.mhit_kernel_timing(self._h, 1 if enable else 0), "kernel_timing")
+
+    def kernel_stats(self):
+        cnt = ctypes.c_int(0)
+        check(self._lib.mhit_kernel_stats(self._h, None, 0, ctypes.byref(cnt)), "kernel_stats")
+        arr = (KStat * max(cnt.value, 1))()
+        check(self._lib.mhit_kernel_stats(self._h, arr, cnt.value, ctypes.byref(cnt)), "kernel_stats")
+        out = []
+        for s in arr[:cnt.value]:
+            out.append({"name": s.name.decode(), "launches": s.launches, "total_ms": s.total_ms,
+                        "avg_ms": s.total_ms / max(s.launches, 1), "bytes": s.bytes_per_launch})
+        return out
+
+    def debug_buffer(self, which):
+        """Intermediate device buffer of the last step (see mhit_debug_buffer)."""
+        n = self.n
+        out = np.empty((n, n, n), dtype=np.float64, order="F")
+        check(self._lib.mhit_debug_buffer(self._h, int(which), out.ctypes.data_as(ctypes.c_void_p)),
+              f"debug_buffer({which})")
+        return out
+
+    def device_bytes(self):
+        b = ctypes.c_size_t(0)
+        check(self._lib.mhit_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
+        return b.value

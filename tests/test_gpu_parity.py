@@ -1,0 +1,78 @@
+"""GPU parity: libmhit.so (HIP, gfx950) against the CPU oracle.
+
+Both run the same IEEE operation sequence (no FMA contraction, the same FFT
+plan, twiddles and reduction order), so the bar is bit-identical fields.  The
+floating-point tolerance that the target states for FP64 fields (1e-10
+relative after N steps) is asserted as well, so a failure reports both.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import run_pair, random_velocity, drops_phi, max_rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10   # FP64 field tolerance after N steps (relative to max |field|)
+
+
+def check_fields(s, o, fields=(1, 2, 3, 4)):
+    for f in fields:
+        got, ref = s.get_field(f), o.get_field(f)
+        assert np.all(np.isfinite(got)), f"field {f} not finite"
+        rel = max_rel(got, ref)
+        assert rel <= TOL, f"field {f}: max rel diff {rel:.3e} > {TOL}"
+        assert np.array_equal(got, ref), f"field {f}: not bit-identical (max rel {rel:.3e})"
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 48, 64, 96])
+def test_tgv_ns_bitwise(n):
+    """C1 shape: Taylor-Green, NS only (main.f90:218-233, 329-946)."""
+    kw = dict(nx=n, dt=1e-3, mu=1.0 / 1600, phiflag=0)
+    s, o, ds, do = run_pair(kw, 5, chunks=[1, 4])
+    check_fields(s, o)
+    assert ds.gumax == do["gumax"] and ds.cou == do["cou"]
+    assert ds.umean == do["umean"] and ds.vmean == do["vmean"] and ds.wmean == do["wmean"]
+    s.close()
+
+
+def test_c1_tgv_64_200_steps():
+    """C1: 64^3 TGV, nu=1/1600, dt=1e-3, Euler then AB2, 200 steps."""
+    kw = dict(nx=64, dt=1e-3, mu=1.0 / 1600, phiflag=0)
+    s, o, ds, do = run_pair(kw, 200, chunks=[1, 99, 100])
+    check_fields(s, o)
+    assert ds.t == 200 and do["t"] == 200
+    s.close()
+
+
+@pytest.mark.parametrize("n,wavenumbers,fp32", [(32, 0, 1), (32, 1, 0), (24, 0, 0)])
+def test_forced_random_ns(n, wavenumbers, fp32):
+    """ABC forcing (A13), random velocity, both wavenumber tables, with and
+    without the REAL(4) Poisson RHS rounding."""
+    kw = dict(nx=n, dt=2e-3, mu=0.005, f1=1.0, f2=1.0, f3=1.0, k0=2.0, phiflag=0,
+              wavenumbers=wavenumbers, poisson_rhs_fp32=fp32, inflow=1)
+    u, v, w = random_velocity(n, seed=1234)
+    s, o, ds, do = run_pair(kw, 10, fields={1: u, 2: v, 3: w}, chunks=[3, 7])
+    check_fields(s, o)
+    s.close()
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_acdi_single_drop(n):
+    """NS + ACDI + CSF with the reference's drop IC (main.f90:263-278)."""
+    kw = dict(nx=n, dt=5e-4, mu=0.01, f1=0.5, f2=0.3, f3=0.2, radius=1.0, sigma=0.1, phiflag=1)
+    s, o, ds, do = run_pair(kw, 6, chunks=[1, 5])
+    check_fields(s, o, fields=(1, 2, 3, 4, 5))
+    s.close()
+
+
+def test_acdi_multi_drop_random():
+    """NS + ACDI with several drops, random velocity, gamma factor 1.2."""
+    n = 48
+    kw = dict(nx=n, dt=5e-4, mu=0.005, f1=1.0, f2=1.0, f3=1.0, radius=0.6, sigma=0.1, phiflag=1,
+              inflow=1, inphi=1, gamma_factor=1.2, epsr=1.0)
+    u, v, w = random_velocity(n, seed=7)
+    dx = 2 * np.pi / (n - 1)
+    phi = drops_phi(n, [(1.5, 1.5, 1.5), (4.5, 4.0, 2.0), (2.0, 4.5, 4.5)], 0.6, dx)
+    s, o, ds, do = run_pair(kw, 8, fields={1: u, 2: v, 3: w, 5: phi}, chunks=[2, 6])
+    check_fields(s, o, fields=(1, 2, 3, 4, 5))
+    s.close()
