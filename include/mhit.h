@@ -145,8 +145,7 @@ int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
 /* Diagnostics (no counterpart in the reference): evaluate the device's own
  * elementary functions on `count` arguments (op 0 log, 1 tanh, 2 sqrt,
  * 3 reciprocal), and copy an intermediate device buffer of the last step
- * (which: 10 psi, 11-13 sharpening flux x/y/z, 14 rhs phi, 15-17 CSF force
- * x/y/z, 18-20 u*, v*, w*, 21-23 rhs_o u/v/w).  Used by the parity tests to
+ * (which: 18-20 u*, v*, w*, 21-23 rhs_o u/v/w).  Used by the parity tests to
  * localise a difference. */
 int mhit_debug_math(int op, size_t count, const double *x, double *y);
 int mhit_debug_buffer(mhit_ctx *ctx, int which, double *host);

@@ -47,10 +47,6 @@ struct Fields {
     double *us, *vs, *ws;        // provisional velocity u*
     double *ruo, *rvo, *rwo;     // rhs of the previous step (AB2)
     double *phi, *phin;          // phase field, current and next
-    double *rphi;                // rhs of phi (convective + diffusive)
-    double *psi;                 // distance function
-    double *sx, *sy, *sz;        // sharpening flux a*n at cell centres
-    double *fx, *fy, *fz;        // CSF force at cell centres
     double *p;                   // pressure
     double2 *spec;               // half spectrum, [k][j][kx], leading dim ld
     double *lsum;                // per x-line partial sums, 3 * n * n
@@ -71,11 +67,7 @@ struct LaunchInfo {
     double bytes;
 };
 
-void launch_phi_rhs(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_psi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_sflux(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_phi_update(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_csf(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta,
                      hipStream_t s, LaunchInfo *li);
 void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li);

@@ -19,128 +19,114 @@ __device__ __forceinline__ bool halted(const Scalars *sc) { return sc->halted !=
 static inline unsigned nblocks(size_t work, int blk) { return (unsigned)((work + blk - 1) / blk); }
 
 // ---------------------------------------------------------------------------
-// Phase field, main.f90:338-486
+// Phase field, main.f90:338-486, fused into one pass (A4-A10).
+//
+// A block owns a TX x TY column of cells and marches KZ planes in z.  Two LDS
+// rings carry what neighbouring cells share:
+//   psi (A6) on a (TX+4) x (TY+4) box for 5 planes, and
+//   the sharpening flux s = a*n (A7-A9, a = 0.25(1 - tanh^2(psi/(2 eps))), n
+//   the normalised undivided normal) on a (TX+2) x (TY+2) box for 3 planes.
+// For output plane k the kernel first adds psi of plane k+2 and s of plane
+// k+1, then forms the convective and diffusive terms (A4, A5) from global
+// memory, the sharpening divergence from the s ring, and writes
+// phi^{n+1} = phi + dt*rhs (A10) to a second buffer.  HBM traffic is one read
+// of u, v, w, phi and one write of phi^{n+1}; psi and s never leave LDS.
 // ---------------------------------------------------------------------------
-// A4 + A5: convective and diffusive terms of rhs phi (main.f90:341-381).
-__global__ void __launch_bounds__(BLK) k_phi_rhs(const double *__restrict__ u, const double *__restrict__ v,
-                                                 const double *__restrict__ w, const double *__restrict__ phi,
-                                                 double *__restrict__ rphi, const Scalars *__restrict__ sc, Consts c)
+constexpr int ATX = 32, ATY = 8;
+constexpr int APX = ATX + 4, APY = ATY + 4;       // psi box
+constexpr int ASX = ATX + 2, ASY = ATY + 2;       // s box
+constexpr int ATHREADS = ATX * ATY;
+
+__device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+
+__global__ void __launch_bounds__(ATHREADS) k_acdi(
+    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+    const double *__restrict__ phi, double *__restrict__ phin, const Scalars *__restrict__ sc, Consts c, int kz)
 {
     if (halted(sc)) return;
+    __shared__ double psiL[5][APY * APX];
+    __shared__ double sL[3][3][ASY * ASX];
     const int n = c.n;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)n * n * n) return;
-    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
-    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
-    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
-    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * ATX, j0 = blockIdx.y * ATY, k0 = blockIdx.z * kz;
+    const double eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
     const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const double gamma = c.gamma_factor * sc->gumax;
-    const double dxi = c.dxi, ddxi = c.ddxi, eps = c.eps;
-    double f0 = phi[t];
-    double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
-    double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
-    double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
-    double uip = u[IDX(ip, j, k)] - um, u0 = u[t] - um;
-    double vjp = v[IDX(i, jp, k)] - vm, v0 = v[t] - vm;
-    double wkp = w[IDX(i, j, kp)] - wm, w0 = w[t] - wm;
-    double A = uip * 0.5 * (fxp + f0) - u0 * 0.5 * (f0 + fxm);
-    double B = vjp * 0.5 * (fyp + f0) - v0 * 0.5 * (f0 + fym);
-    double C = wkp * 0.5 * (fzp + f0) - w0 * 0.5 * (f0 + fzm);
-    double r = (-(A * dxi) - B * dxi) - C * dxi;
-    double dxx = eps * ((fxp - 2.0 * f0) + fxm) * ddxi;
-    double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;
-    double dzz = eps * ((fzp - 2.0 * f0) + fzm) * ddxi;
-    rphi[t] = r + gamma * ((dxx + dyy) + dzz);
-}
-
-// A6: psi = eps*log((min(phi,1)+enum)/(1-min(phi,1)+enum)) (main.f90:384-393).
-__global__ void __launch_bounds__(BLK) k_psi(const double *__restrict__ phi, double *__restrict__ psi,
-                                             const Scalars *__restrict__ sc, Consts c)
-{
-    if (halted(sc)) return;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)c.n * c.n * c.n) return;
-    double f = phi[t];
-    double val = f < 1.0 ? f : 1.0;
-    psi[t] = c.eps * dlog((val + c.enumr) / ((1.0 - val) + c.enumr));
-}
-
-// A7 + A8 and the per-cell factor of A9: normalised undivided normals times
-// a = 0.25*(1 - tanh^2(0.5*psi*epsi)) (main.f90:397-465).
-__global__ void __launch_bounds__(BLK) k_sflux(const double *__restrict__ psi, double *__restrict__ sx,
-                                               double *__restrict__ sy, double *__restrict__ sz,
-                                               const Scalars *__restrict__ sc, Consts c)
-{
-    if (halted(sc)) return;
-    const int n = c.n;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)n * n * n) return;
-    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
-    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
-    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
-    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
     const double tiny = (double)1.0E-16f;
-    double gx = psi[IDX(ip, j, k)] - psi[IDX(im, j, k)];
-    double gy = psi[IDX(i, jp, k)] - psi[IDX(i, jm, k)];
-    double gz = psi[IDX(i, j, kp)] - psi[IDX(i, j, km)];
-    double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-    double th = dtanh((0.5 * psi[t]) * c.epsi);
-    double a = 0.25 * (1.0 - th * th);
-    sx[t] = a * (gx * normod);
-    sy[t] = a * (gy * normod);
-    sz[t] = a * (gz * normod);
-}
 
-// A9 + A10: sharpening flux divergence and Euler update of phi (main.f90:446-480).
-__global__ void __launch_bounds__(BLK) k_phi_update(const double *__restrict__ phi, const double *__restrict__ rphi,
-                                                    const double *__restrict__ sx, const double *__restrict__ sy,
-                                                    const double *__restrict__ sz, double *__restrict__ phin,
-                                                    const Scalars *__restrict__ sc, Consts c)
-{
-    if (halted(sc)) return;
-    const int n = c.n;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)n * n * n) return;
-    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
-    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
-    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
-    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
-    const double gamma = c.gamma_factor * sc->gumax;
-    double X = (sx[IDX(ip, j, k)] - sx[IDX(im, j, k)]) * 0.5 * c.dxi;
-    double Y = (sy[IDX(i, jp, k)] - sy[IDX(i, jm, k)]) * 0.5 * c.dxi;
-    double Z = (sz[IDX(i, j, kp)] - sz[IDX(i, j, km)]) * 0.5 * c.dxi;
-    double r = rphi[t] - gamma * ((X + Y) + Z);
-    phin[t] = phi[t] + c.dt * r;
-}
+    auto psi_plane = [&](int m) {           // A6 on plane m (any integer)
+        const int slot = (m - k0 + 2) % 5;
+        const int gk = wrap(m, n);
+        for (int e = tid; e < APX * APY; e += ATHREADS) {
+            int bx = e % APX, by = e / APX;
+            int gi = wrap(i0 - 2 + bx, n), gj = wrap(j0 - 2 + by, n);
+            double f = phi[IDX(gi, gj, gk)];
+            double val = f < 1.0 ? f : 1.0;
+            psiL[slot][e] = eps * dlog((val + en) / ((1.0 - val) + en));
+        }
+    };
+    auto s_plane = [&](int m) {             // A7 + A8 + a(psi) on plane m
+        const int ps = (m - k0 + 2) % 5, psm = (m - 1 - k0 + 2) % 5, psp = (m + 1 - k0 + 2) % 5;
+        const int slot = (m - k0 + 1) % 3;
+        for (int e = tid; e < ASX * ASY; e += ATHREADS) {
+            int sx = e % ASX, sy = e / ASX;
+            int pc = (sy + 1) * APX + (sx + 1);
+            double gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
+            double gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
+            double gz = psiL[psp][pc] - psiL[psm][pc];
+            double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+            double th = dtanh((0.5 * psiL[ps][pc]) * epsi);
+            double a = 0.25 * (1.0 - th * th);
+            sL[slot][0][e] = a * (gx * normod);
+            sL[slot][1][e] = a * (gy * normod);
+            sL[slot][2][e] = a * (gz * normod);
+        }
+    };
 
-// A14, first half: CSF force at cell centres from phi^{n+1} (main.f90:604-627).
-__global__ void __launch_bounds__(BLK) k_csf(const double *__restrict__ phi, double *__restrict__ fx,
-                                             double *__restrict__ fy, double *__restrict__ fz,
-                                             const Scalars *__restrict__ sc, Consts c)
-{
-    if (halted(sc)) return;
-    const int n = c.n;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)n * n * n) return;
-    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
-    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
-    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
-    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
-    const double sig6 = 6.0 * c.sigma;
-    double f0 = phi[t];
-    double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
-    double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
-    double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
-    double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * c.epsi;
-    double s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - 6.0 * f0;
-    double chem = t1 - c.eps * s6 * c.ddxi;
-    double gx = 0.5 * (fxp - fxm) * c.dxi;
-    double gy = 0.5 * (fyp - fym) * c.dxi;
-    double gz = 0.5 * (fzp - fzm) * c.dxi;
-    fx[t] = sig6 * chem * gx;
-    fy[t] = sig6 * chem * gy;
-    fz[t] = sig6 * chem * gz;
+    // prologue: psi for planes k0-2 .. k0+1, s for planes k0-1, k0
+    for (int m = k0 - 2; m <= k0 + 1; m++) psi_plane(m);
+    __syncthreads();
+    s_plane(k0 - 1);
+    s_plane(k0);
+
+    const int tx = tid % ATX, ty = tid / ATX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = (i < n) && (j < n);
+    const int ip = i + 1 >= n ? i + 1 - n : i + 1, im = i == 0 ? n - 1 : i - 1;
+    const int jp = j + 1 >= n ? j + 1 - n : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    const int kend = min(k0 + kz, n);
+    for (int k = k0; k < kend; k++) {
+        psi_plane(k + 2);
+        __syncthreads();
+        s_plane(k + 1);
+        __syncthreads();
+        if (active) {
+            const int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+            const size_t t = IDX(i, j, k);
+            double f0 = phi[t];
+            double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
+            double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
+            double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
+            double uip = u[IDX(ip, j, k)] - um, u0 = u[t] - um;
+            double vjp = v[IDX(i, jp, k)] - vm, v0 = v[t] - vm;
+            double wkp = w[IDX(i, j, kp)] - wm, w0 = w[t] - wm;
+            double A = uip * 0.5 * (fxp + f0) - u0 * 0.5 * (f0 + fxm);
+            double B = vjp * 0.5 * (fyp + f0) - v0 * 0.5 * (f0 + fym);
+            double C = wkp * 0.5 * (fzp + f0) - w0 * 0.5 * (f0 + fzm);
+            double r = (-(A * dxi) - B * dxi) - C * dxi;
+            double dxx = eps * ((fxp - 2.0 * f0) + fxm) * ddxi;
+            double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;
+            double dzz = eps * ((fzp - 2.0 * f0) + fzm) * ddxi;
+            double rphi = r + gamma * ((dxx + dyy) + dzz);
+            const int s0 = (k - k0 + 1) % 3, sm = (k - 1 - k0 + 1) % 3, sp = (k + 1 - k0 + 1) % 3;
+            const int sc0 = (ty + 1) * ASX + (tx + 1);
+            double X = (sL[s0][0][sc0 + 1] - sL[s0][0][sc0 - 1]) * 0.5 * dxi;
+            double Y = (sL[s0][1][sc0 + ASX] - sL[s0][1][sc0 - ASX]) * 0.5 * dxi;
+            double Z = (sL[sp][2][sc0] - sL[sm][2][sc0]) * 0.5 * dxi;
+            double r2 = rphi - gamma * ((X + Y) + Z);
+            phin[t] = f0 + c.dt * r2;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +135,7 @@ __global__ void __launch_bounds__(BLK) k_csf(const double *__restrict__ phi, dou
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BLK) k_momentum(
     const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ fx, const double *__restrict__ fy, const double *__restrict__ fz,
+    const double *__restrict__ phi,
     double *__restrict__ us, double *__restrict__ vs, double *__restrict__ ws,
     double *__restrict__ ruo, double *__restrict__ rvo, double *__restrict__ rwo,
     const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
@@ -209,9 +195,32 @@ __global__ void __launch_bounds__(BLK) k_momentum(
     rv = (rv + c.f1 * sinf[i]) + c.f3 * cosf[k];
     rw = (rw + c.f2 * sinf[j]) + c.f1 * cosf[i];
     if (c.phiflag) {
-        ru = ru + 0.5 * (fx[IDX(im, j, k)] + fx[t]) * rhoi;
-        rv = rv + 0.5 * (fy[IDX(i, jm, k)] + fy[t]) * rhoi;
-        rw = rw + 0.5 * (fz[IDX(i, j, km)] + fz[t]) * rhoi;
+        // A14: CSF force at the cell centres (i,j,k), (i-1,j,k), (i,j-1,k),
+        // (i,j,k-1) from phi^{n+1}, interpolated to the faces.
+        const int imm = im == 0 ? n - 1 : im - 1, jmm = jm == 0 ? n - 1 : jm - 1, kmm = km == 0 ? n - 1 : km - 1;
+        const double sig6 = 6.0 * c.sigma, eps = c.eps, epsi = c.epsi;
+        auto chem = [&](int a, int b, int cc, int ap, int am, int bp, int bm, int cp, int cm, double f0) {
+            double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * epsi;
+            double s6 = (((((phi[IDX(ap, b, cc)] + phi[IDX(am, b, cc)]) + phi[IDX(a, bp, cc)]) + phi[IDX(a, bm, cc)]) +
+                          phi[IDX(a, b, cp)]) + phi[IDX(a, b, cm)]) - 6.0 * f0;
+            return t1 - eps * s6 * ddxi;
+        };
+        const double p0 = phi[t], pxp = phi[IDX(ip, j, k)], pxm = phi[IDX(im, j, k)];
+        const double pyp = phi[IDX(i, jp, k)], pym = phi[IDX(i, jm, k)];
+        const double pzp = phi[IDX(i, j, kp)], pzm = phi[IDX(i, j, km)];
+        const double c0 = chem(i, j, k, ip, im, jp, jm, kp, km, p0);
+        const double cxm = chem(im, j, k, i, imm, jp, jm, kp, km, pxm);
+        const double cym = chem(i, jm, k, ip, im, j, jmm, kp, km, pym);
+        const double czm = chem(i, j, km, ip, im, jp, jm, k, kmm, pzm);
+        const double fx0 = sig6 * c0 * (0.5 * (pxp - pxm) * dxi);
+        const double fy0 = sig6 * c0 * (0.5 * (pyp - pym) * dxi);
+        const double fz0 = sig6 * c0 * (0.5 * (pzp - pzm) * dxi);
+        const double fxm = sig6 * cxm * (0.5 * (p0 - phi[IDX(imm, j, k)]) * dxi);
+        const double fym = sig6 * cym * (0.5 * (p0 - phi[IDX(i, jmm, k)]) * dxi);
+        const double fzm = sig6 * czm * (0.5 * (p0 - phi[IDX(i, j, kmm)]) * dxi);
+        ru = ru + 0.5 * (fxm + fx0) * rhoi;
+        rv = rv + 0.5 * (fym + fy0) * rhoi;
+        rw = rw + 0.5 * (fzm + fz0) * rhoi;
     }
     const double dt = c.dt;
     us[t] = u0 + dt * (alpha * ru - beta * ruo[t]);
@@ -225,10 +234,12 @@ __global__ void __launch_bounds__(BLK) k_momentum(
 // ---------------------------------------------------------------------------
 // FFT machinery: mixed-radix Stockham autosort in LDS (Govindaraju et al.,
 // SC'08), unnormalised.  A block holds `nlines` lines of length n; element p of
-// line l sits at lds[l*lstride + p*pstride].
+// line l sits at lds[l*lstride + p*pstride].  Each thread owns at most one
+// radix-R butterfly per stage (the block has >= nlines*n/Rmin threads);
+// consecutive threads take consecutive lines so that LDS accesses of a
+// wavefront are contiguous for column tiles.
 // ---------------------------------------------------------------------------
-constexpr int FFT_THREADS = 256;
-constexpr int MAXB = 4;   // butterflies per thread per stage
+constexpr int FFT_MAX_THREADS = 1024;
 constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block
 
 template <int R, int SG>
@@ -237,39 +248,34 @@ __device__ __forceinline__ void fft_stage(double2 *lds, int n, int nlines, int l
 {
     const int L = n / R;
     const int total = nlines * L;
-    double2 v[MAXB][R];
+    const int t = threadIdx.x;
+    const bool active = t < total;
+    double2 v[R];
+    int line = 0, j = 0;
+    if (active) {
+        line = t % nlines;
+        j = t / nlines;
+        const double2 *base = lds + line * lstride;
 #pragma unroll
-    for (int b = 0; b < MAXB; b++) {
-        int t = threadIdx.x + b * FFT_THREADS;
-        if (t < total) {
-            int line = t / L, j = t - line * L;
-            const double2 *base = lds + line * lstride;
-#pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = base[(j + r * L) * pstride];
-        }
+        for (int r = 0; r < R; r++) v[r] = base[(j + r * L) * pstride];
     }
     __syncthreads();
+    if (active) {
+        const int k = j % Ns;
+        if (!first) {
 #pragma unroll
-    for (int b = 0; b < MAXB; b++) {
-        int t = threadIdx.x + b * FFT_THREADS;
-        if (t < total) {
-            int line = t / L, j = t - line * L;
-            int k = j % Ns;
-            if (!first) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    double2 wv = tw[k * (R - 1) + r - 1];
-                    double wi = SG < 0 ? -wv.y : wv.y;
-                    double xr = v[b][r].x, xi = v[b][r].y;
-                    v[b][r] = make_double2(xr * wv.x - xi * wi, xr * wi + xi * wv.x);
-                }
+            for (int r = 1; r < R; r++) {
+                double2 wv = tw[k * (R - 1) + r - 1];
+                double wi = SG < 0 ? -wv.y : wv.y;
+                double xr = v[r].x, xi = v[r].y;
+                v[r] = make_double2(xr * wv.x - xi * wi, xr * wi + xi * wv.x);
             }
-            dft<R, SG>(v[b]);
-            int base_idx = (j / Ns) * Ns * R + k;
-            double2 *base = lds + line * lstride;
-#pragma unroll
-            for (int r = 0; r < R; r++) base[(base_idx + r * Ns) * pstride] = v[b][r];
         }
+        dft<R, SG>(v);
+        const int base_idx = (j / Ns) * Ns * R + k;
+        double2 *base = lds + line * lstride;
+#pragma unroll
+        for (int r = 0; r < R; r++) base[(base_idx + r * Ns) * pstride] = v[r];
     }
     __syncthreads();
 }
@@ -280,7 +286,7 @@ __device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, cons
 {
     for (int s = 0; s < pl.nst; s++) {
         const double2 *t = tw + pl.twoff[s];
-        bool first = (s == 0);
+        const bool first = (s == 0);
         switch (pl.R[s]) {
         case 8: fft_stage<8, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
         case 4: fft_stage<4, SG>(lds, pl.n, nlines, lstride, pstride, pl.Ns[s], t, first); break;
@@ -293,7 +299,7 @@ __device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, cons
 // A16 + A17 + forward x pass: divergence of u*, REAL(4) rounding of the
 // Poisson RHS, then one complex FFT per pair of x-lines (j, j+1) whose
 // spectrum is split into the two half spectra (main.f90:715-750).
-__global__ void __launch_bounds__(FFT_THREADS) k_xfwd(
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
     const double *__restrict__ us, const double *__restrict__ vs, const double *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
     Consts c, FftPlan pl, int ppb, int ld)
@@ -304,7 +310,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_xfwd(
     const int lstride = n + 1;
     const int pair0 = blockIdx.x * ppb;
     const double cdiv = c.rho * c.dxi / c.dt;
-    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < ppb * n; e += blockDim.x) {
         int pl_ = e / n, i = e - pl_ * n;
         int pair = pair0 + pl_;
         int k = pair / half, j = 2 * (pair - k * half);
@@ -326,7 +332,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_xfwd(
     }
     __syncthreads();
     fft_lds<-1>(lds, ppb, lstride, 1, pl, tw);
-    for (int e = threadIdx.x; e < ppb * nh; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < ppb * nh; e += blockDim.x) {
         int pl_ = e / nh, q = e - pl_ * nh;
         int pair = pair0 + pl_;
         int k = pair / half, j = 2 * (pair - k * half);
@@ -340,8 +346,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_xfwd(
 
 // Column pass along y (forward or inverse), in place.  Block = (kx tile, k).
 template <int SG>
-__global__ void __launch_bounds__(FFT_THREADS) k_ypass(double2 *__restrict__ spec, const double2 *__restrict__ tw,
-                                                       const Scalars *__restrict__ sc, FftPlan pl, int ld, int tkx)
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__ spec,
+                                                           const double2 *__restrict__ tw,
+                                                           const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                           int tkx)
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
@@ -349,13 +357,13 @@ __global__ void __launch_bounds__(FFT_THREADS) k_ypass(double2 *__restrict__ spe
     const int q0 = blockIdx.x * tkx, k = blockIdx.y;
     const int nq = min(tkx, nh - q0);
     double2 *base = spec + (size_t)k * n * ld + q0;
-    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         lds[r * tkx + ql] = ql < nq ? base[(size_t)r * ld + ql] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lds<SG>(lds, tkx, 1, tkx, pl, tw);
-    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         if (ql < nq) base[(size_t)r * ld + ql] = lds[r * tkx + ql];
     }
@@ -363,10 +371,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_ypass(double2 *__restrict__ spe
 
 // Column pass along z: forward FFT, division by -k^2 and N^3 with the mean
 // mode set to zero (A19, main.f90:765-798), inverse FFT.  Block = (kx tile, j).
-__global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ spec, const double2 *__restrict__ tw,
-                                                          const double *__restrict__ ktab,
-                                                          const Scalars *__restrict__ sc, FftPlan pl, int ld,
-                                                          int tkx)
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restrict__ spec,
+                                                              const double2 *__restrict__ tw,
+                                                              const double *__restrict__ ktab,
+                                                              const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                              int tkx)
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
@@ -375,7 +384,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ 
     const int nq = min(tkx, nh - q0);
     const size_t rs = (size_t)n * ld;
     double2 *base = spec + (size_t)j * ld + q0;
-    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         lds[r * tkx + ql] = ql < nq ? base[(size_t)r * rs + ql] : make_double2(0.0, 0.0);
     }
@@ -383,7 +392,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ 
     fft_lds<-1>(lds, tkx, 1, tkx, pl, tw);
     const double nnn = (double)((long long)n * (long long)n * (long long)n);
     const double ky = ktab[j];
-    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         int q = q0 + ql;
         if (ql < nq) {
@@ -397,7 +406,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ 
     }
     __syncthreads();
     fft_lds<1>(lds, tkx, 1, tkx, pl, tw);
-    for (int e = threadIdx.x; e < n * tkx; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         if (ql < nq) base[(size_t)r * rs + ql] = lds[r * tkx + ql];
     }
@@ -405,16 +414,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_zpoisson(double2 *__restrict__ 
 
 // Inverse x pass: Hermitian extension of the half spectra of lines j and j+1,
 // one complex inverse FFT per pair, p(.,j) = Re, p(.,j+1) = Im (A20-A21).
-__global__ void __launch_bounds__(FFT_THREADS) k_xinv(const double2 *__restrict__ spec, double *__restrict__ p,
-                                                      const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
-                                                      FftPlan pl, int ppb, int ld)
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restrict__ spec,
+                                                          double *__restrict__ p, const double2 *__restrict__ tw,
+                                                          const Scalars *__restrict__ sc, FftPlan pl, int ppb,
+                                                          int ld)
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
     const int n = pl.n, half = n / 2;
     const int lstride = n + 1;
     const int pair0 = blockIdx.x * ppb;
-    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < ppb * n; e += blockDim.x) {
         int pl_ = e / n, q = e - pl_ * n;
         int pair = pair0 + pl_;
         int k = pair / half, j = 2 * (pair - k * half);
@@ -435,7 +445,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_xinv(const double2 *__restrict_
     }
     __syncthreads();
     fft_lds<1>(lds, ppb, lstride, 1, pl, tw);
-    for (int e = threadIdx.x; e < ppb * n; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < ppb * n; e += blockDim.x) {
         int pl_ = e / n, i = e - pl_ * n;
         int pair = pair0 + pl_;
         int k = pair / half, j = 2 * (pair - k * half);
@@ -599,72 +609,67 @@ void launch_debug_math(int op, size_t count, const double *x, double *y, hipStre
 
 static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
 
-void launch_phi_rhs(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+static int acdi_kz(int n)
 {
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_phi_rhs, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.phi, f.rphi, f.sc, c);
-    li->name = "phi_rhs"; li->bytes = 5 * F(c);
+    for (int kz = 32; kz > 1; kz >>= 1)
+        if (n % kz == 0 && (n / kz) * ((n + ATX - 1) / ATX) * ((n + ATY - 1) / ATY) >= 2048) return kz;
+    for (int kz = 8; kz > 1; kz >>= 1)
+        if (n % kz == 0) return kz;
+    return 1;
 }
 
-void launch_psi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
 {
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_psi, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.psi, f.sc, c);
-    li->name = "psi"; li->bytes = 2 * F(c);
-}
-
-void launch_sflux(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
-{
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_sflux, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.psi, f.sx, f.sy, f.sz, f.sc, c);
-    li->name = "sflux"; li->bytes = 4 * F(c);
-}
-
-void launch_phi_update(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
-{
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_phi_update, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.rphi, f.sx, f.sy, f.sz,
-                       f.phin, f.sc, c);
-    li->name = "phi_update"; li->bytes = 6 * F(c);
-}
-
-void launch_csf(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
-{
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_csf, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.phi, f.fx, f.fy, f.fz, f.sc, c);
-    li->name = "csf"; li->bytes = 4 * F(c);
+    const int n = c.n, kz = acdi_kz(n);
+    dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, n / kz);
+    hipLaunchKernelGGL(k_acdi, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, c, kz);
+    li->name = "acdi"; li->bytes = 5 * F(c);
 }
 
 void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta, hipStream_t s,
                      LaunchInfo *li)
 {
     size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_momentum, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.fx, f.fy, f.fz, f.us,
+    hipLaunchKernelGGL(k_momentum, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.phi, f.us,
                        f.vs, f.ws, f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta);
-    li->name = "momentum"; li->bytes = (c.phiflag ? 15 : 12) * F(c);
+    li->name = "momentum"; li->bytes = (c.phiflag ? 13 : 12) * F(c);
 }
 
-static int pairs_per_block(const FftPlan &pl)
+static int plan_rmin(const FftPlan &pl)
 {
-    // largest power of two p with p*n/Rmin <= MAXB*FFT_THREADS and p <= 8
     int rmin = 8;
     for (int s = 0; s < pl.nst; s++) rmin = pl.R[s] < rmin ? pl.R[s] : rmin;
+    return rmin;
+}
+
+// Lines per block for the x passes: a power of two p <= 8 with one butterfly
+// per thread (p*n/Rmin <= 1024 threads), LDS within LDS_CAP, dividing n*n/2.
+static int pairs_per_block(const FftPlan &pl)
+{
+    const int rmin = plan_rmin(pl);
     int p = 8;
-    while (p > 1 && (p * (pl.n / rmin) > MAXB * FFT_THREADS)) p >>= 1;
+    while (p > 1 && p * (pl.n / rmin) > FFT_MAX_THREADS) p >>= 1;
     while (p > 1 && (size_t)p * (pl.n + 1) * sizeof(double2) > LDS_CAP) p >>= 1;
-    int pairs = pl.n * pl.n / 2;
+    const int pairs = pl.n * pl.n / 2;
     while (p > 1 && pairs % p != 0) p >>= 1;
     return p;
 }
 
+// kx columns per block for the y/z passes.
 static int col_tile(const FftPlan &pl)
 {
-    int rmin = 8;
-    for (int s = 0; s < pl.nst; s++) rmin = pl.R[s] < rmin ? pl.R[s] : rmin;
+    const int rmin = plan_rmin(pl);
     int t = 8;
-    while (t > 1 && (t * (pl.n / rmin) > MAXB * FFT_THREADS)) t >>= 1;
+    while (t > 1 && t * (pl.n / rmin) > FFT_MAX_THREADS) t >>= 1;
     while (t > 1 && (size_t)t * pl.n * sizeof(double2) > LDS_CAP) t >>= 1;
     return t;
+}
+
+static int fft_threads(const FftPlan &pl, int nlines)
+{
+    int t = nlines * (pl.n / plan_rmin(pl));
+    t = ((t + 63) / 64) * 64;
+    return t < 64 ? 64 : t;
 }
 
 void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
@@ -672,8 +677,8 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
     int ppb = pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
-    hipLaunchKernelGGL(k_xfwd, dim3(pairs / ppb), dim3(FFT_THREADS), shm, s, f.us, f.vs, f.ws, f.spec, f.tw, f.sc,
-                       c, pl, ppb, f.ld);
+    hipLaunchKernelGGL(k_xfwd, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, f.us, f.vs, f.ws, f.spec,
+                       f.tw, f.sc, c, pl, ppb, f.ld);
     li->name = "xfwd_div"; li->bytes = 4 * F(c);
 }
 
@@ -684,10 +689,11 @@ void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inver
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
     size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+    dim3 blk(fft_threads(pl, tkx));
     if (inverse)
-        hipLaunchKernelGGL(k_ypass<1>, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        hipLaunchKernelGGL(k_ypass<1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
     else
-        hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        hipLaunchKernelGGL(k_ypass<-1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
     li->name = inverse ? "y_inv" : "y_fwd";
     li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
 }
@@ -698,7 +704,8 @@ void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStr
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
     size_t shm = sizeof(double2) * (size_t)tkx * c.n;
-    hipLaunchKernelGGL(k_zpoisson, grid, dim3(FFT_THREADS), shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+    hipLaunchKernelGGL(k_zpoisson, grid, dim3(fft_threads(pl, tkx)), shm, s, f.spec, f.tw, f.ktab, f.sc, pl,
+                       f.ld, tkx);
     li->name = "z_poisson";
     li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
 }
@@ -708,8 +715,8 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
     int ppb = pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
-    hipLaunchKernelGGL(k_xinv, dim3(pairs / ppb), dim3(FFT_THREADS), shm, s, f.spec, f.p, f.tw, f.sc, pl, ppb,
-                       f.ld);
+    hipLaunchKernelGGL(k_xinv, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, f.spec, f.p, f.tw, f.sc, pl,
+                       ppb, f.ld);
     li->name = "x_inv"; li->bytes = 16.0 * (double)(c.n / 2 + 1) * c.n * c.n + F(c);
 }
 

@@ -308,7 +308,7 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     double **need[] = {&f.u, &f.v, &f.w, &f.us, &f.vs, &f.ws, &f.ruo, &f.rvo, &f.rwo, &f.p};
     for (double **q : need) TRY(dalloc(c, (void **)q, F));
     if (p->phiflag) {
-        double **ph[] = {&f.phi, &f.phin, &f.rphi, &f.psi, &f.sx, &f.sy, &f.sz, &f.fx, &f.fy, &f.fz};
+        double **ph[] = {&f.phi, &f.phin};
         for (double **q : ph) TRY(dalloc(c, (void **)q, F));
     }
     TRY(dalloc(c, (void **)&f.spec, sizeof(double2) * (size_t)n * n * f.ld));
@@ -523,12 +523,8 @@ static void enqueue_step(mhit_ctx *c)
     const double alpha = c->nsteps_since_init == 0 ? 1.0 : 1.5;
     const double beta = c->nsteps_since_init == 0 ? 0.0 : 0.5;
     if (k.phiflag) {
-        timed(c, [&](LaunchInfo *li) { launch_phi_rhs(f, k, s, li); });
-        timed(c, [&](LaunchInfo *li) { launch_psi(f, k, s, li); });
-        timed(c, [&](LaunchInfo *li) { launch_sflux(f, k, s, li); });
-        timed(c, [&](LaunchInfo *li) { launch_phi_update(f, k, s, li); });
+        timed(c, [&](LaunchInfo *li) { launch_acdi(f, k, s, li); });
         std::swap(f.phi, f.phin);
-        timed(c, [&](LaunchInfo *li) { launch_csf(f, k, s, li); });
     }
     timed(c, [&](LaunchInfo *li) { launch_momentum(f, k, alpha, beta, s, li); });
     timed(c, [&](LaunchInfo *li) { launch_xfwd(f, k, c->plan, s, li); });
@@ -656,14 +652,6 @@ extern "C" int mhit_debug_buffer(mhit_ctx *c, int which, double *host)
     Fields &f = c->f;
     double *d = nullptr;
     switch (which) {
-    case 10: d = f.psi; break;
-    case 11: d = f.sx; break;
-    case 12: d = f.sy; break;
-    case 13: d = f.sz; break;
-    case 14: d = f.rphi; break;
-    case 15: d = f.fx; break;
-    case 16: d = f.fy; break;
-    case 17: d = f.fz; break;
     case 18: d = f.us; break;
     case 19: d = f.vs; break;
     case 20: d = f.ws; break;

@@ -76,3 +76,22 @@ def test_acdi_multi_drop_random():
     s, o, ds, do = run_pair(kw, 8, fields={1: u, 2: v, 3: w, 5: phi}, chunks=[2, 6])
     check_fields(s, o, fields=(1, 2, 3, 4, 5))
     s.close()
+
+
+def test_device_math_bitwise():
+    """The device log/tanh (ACDI psi and sharpening factor), sqrt and
+    division round exactly like the oracle's host versions."""
+    from mhit36_amd.solver import debug_math
+    from oracle import orc
+    rng = np.random.default_rng(0)
+    pos = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.5, 2, 20000),
+                          [1e-310, 5e-324, 1.0, 2.0, np.inf]])
+    sym = np.concatenate([rng.uniform(-25, 25, 20000), rng.uniform(-1e-3, 1e-3, 5000), [0.0, 22.5, -30.0]])
+    got = debug_math(0, pos)
+    ref = np.array([orc.log(float(x)) for x in pos])
+    assert np.array_equal(got.view(np.int64), ref.view(np.int64))
+    got = debug_math(1, sym)
+    ref = np.array([orc.tanh(float(x)) for x in sym])
+    assert np.array_equal(got.view(np.int64), ref.view(np.int64))
+    assert np.array_equal(debug_math(2, pos), np.sqrt(pos))
+    assert np.array_equal(debug_math(3, sym[sym != 0]), 1.0 / sym[sym != 0])
