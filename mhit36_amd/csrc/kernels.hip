@@ -18,6 +18,18 @@ __device__ __forceinline__ bool halted(const Scalars *sc) { return sc->halted !=
 
 static inline unsigned nblocks(size_t work, int blk) { return (unsigned)((work + blk - 1) / blk); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumber them so that XCD x works on the x-th contiguous eighth of the grid:
+// neighbouring tiles, which share halo cells, then meet in the same L2.
+__device__ __forceinline__ uint3 xcd_block()
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned nb = gx * gy * gz;
+    unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+    return make_uint3(b % gx, (b / gx) % gy, b / (gx * gy));
+}
+
 // ---------------------------------------------------------------------------
 // Phase field, main.f90:338-486, fused into one pass (A4-A10).
 //
@@ -36,6 +48,7 @@ constexpr int ATX = 32, ATY = 8;
 constexpr int APX = ATX + 4, APY = ATY + 4;       // psi box
 constexpr int ASX = ATX + 2, ASY = ATY + 2;       // s box
 constexpr int ATHREADS = ATX * ATY;
+static_assert(APX * APY <= 2 * ATHREADS, "psi box loads assume <= 2 elements per thread");
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
@@ -48,21 +61,34 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
     __shared__ double sL[3][3][ASY * ASX];
     const int n = c.n;
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * ATX, j0 = blockIdx.y * ATY, k0 = blockIdx.z * kz;
+    const uint3 blk = xcd_block();
+    const int i0 = blk.x * ATX, j0 = blk.y * ATY, k0 = blk.z * kz;
     const double eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
     const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const double gamma = c.gamma_factor * sc->gumax;
     const double tiny = (double)1.0E-16f;
 
+    // The psi box element(s) of this thread and their x/y offsets, fixed for
+    // every plane (at most two per thread: APX*APY <= 2*ATHREADS).
+    size_t poff[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        int e = tid + q * ATHREADS;
+        int bx = e % APX, by = e / APX;
+        poff[q] = (size_t)wrap(i0 - 2 + bx, n) + (size_t)n * wrap(j0 - 2 + by, n);
+    }
+    const size_t nn = (size_t)n * n;
     auto psi_plane = [&](int m) {           // A6 on plane m (any integer)
         const int slot = (m - k0 + 2) % 5;
-        const int gk = wrap(m, n);
-        for (int e = tid; e < APX * APY; e += ATHREADS) {
-            int bx = e % APX, by = e / APX;
-            int gi = wrap(i0 - 2 + bx, n), gj = wrap(j0 - 2 + by, n);
-            double f = phi[IDX(gi, gj, gk)];
-            double val = f < 1.0 ? f : 1.0;
-            psiL[slot][e] = eps * dlog((val + en) / ((1.0 - val) + en));
+        const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            int e = tid + q * ATHREADS;
+            if (e < APX * APY) {
+                double f = phi[poff[q] + zoff];
+                double val = f < 1.0 ? f : 1.0;
+                psiL[slot][e] = eps * dlog((val + en) / ((1.0 - val) + en));
+            }
         }
     };
     auto s_plane = [&](int m) {             // A7 + A8 + a(psi) on plane m
@@ -131,104 +157,181 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
 
 // ---------------------------------------------------------------------------
 // Momentum, main.f90:513-681: A11 convection, A12 viscous, A13 ABC forcing,
-// A14 CSF at the faces, A15 AB2/Euler advance and rhs_o <- rhs.
+// A14 CSF force (phi^{n+1}) interpolated to the faces, A15 AB2/Euler advance
+// and rhs_o <- rhs, in one pass.
+//
+// A block owns an MTX x MTY column and marches kz planes in z.  LDS rings
+// hold u, v, w on an (MTX+2) x (MTY+2) box (planes k-1..k+1), phi on an
+// (MTX+3) x (MTY+3) box (planes k-1..k+1) and the
+// CSF cell-centre force on an (MTX+1) x (MTY+1) box for planes k-1 and k, so
+// every input is read from HBM once per block (plus the halo) and every CSF
+// force is formed once.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLK) k_momentum(
+constexpr int MTX = 32, MTY = 8, MTHREADS = MTX * MTY;
+constexpr int MUX = MTX + 2, MUY = MTY + 2;       // u/v/w box
+constexpr int MPX = MTX + 3, MPY = MTY + 3;       // phi box
+constexpr int MFX = MTX + 1, MFY = MTY + 1;       // force box
+static_assert(MUX * MUY <= 2 * MTHREADS && MPX * MPY <= 2 * MTHREADS, "box loads assume <= 2 elements per thread");
+
+template <int PHI>
+__global__ void __launch_bounds__(MTHREADS) k_momentum(
     const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
     const double *__restrict__ phi,
     double *__restrict__ us, double *__restrict__ vs, double *__restrict__ ws,
     double *__restrict__ ruo, double *__restrict__ rvo, double *__restrict__ rwo,
     const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
-    Consts c, double alpha, double beta)
+    Consts c, double alpha, double beta, int kz)
 {
     if (halted(sc)) return;
+    __shared__ double UL[3][3][MUY * MUX];
+    __shared__ double PL[PHI ? 3 : 1][PHI ? MPY * MPX : 1];
+    __shared__ double FL[PHI ? 2 : 1][3][PHI ? MFY * MFX : 1];
     const int n = c.n;
-    size_t t = (size_t)blockIdx.x * BLK + threadIdx.x;
-    if (t >= (size_t)n * n * n) return;
-    int i = (int)(t % n), j = (int)((t / n) % n), k = (int)(t / ((size_t)n * n));
-    int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
-    int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
-    int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const int tid = threadIdx.x;
+    const uint3 blk = xcd_block();
+    const int i0 = blk.x * MTX, j0 = blk.y * MTY, k0 = blk.z * kz;
     const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const double dxi = c.dxi, ddxi = c.ddxi, mu = c.mu, rhoi = c.rhoi;
-#define U(a, b, cc) (u[IDX(a, b, cc)] - um)
-#define V(a, b, cc) (v[IDX(a, b, cc)] - vm)
-#define W(a, b, cc) (w[IDX(a, b, cc)] - wm)
-    const double u0 = U(i, j, k), v0 = V(i, j, k), w0 = W(i, j, k);
-    const double uxp = U(ip, j, k), uxm = U(im, j, k), uyp = U(i, jp, k), uym = U(i, jm, k);
-    const double uzp = U(i, j, kp), uzm = U(i, j, km);
-    const double vxp = V(ip, j, k), vxm = V(im, j, k), vyp = V(i, jp, k), vym = V(i, jm, k);
-    const double vzp = V(i, j, kp), vzm = V(i, j, km);
-    const double wxp = W(ip, j, k), wxm = W(im, j, k), wyp = W(i, jp, k), wym = W(i, jm, k);
-    const double wzp = W(i, j, kp), wzm = W(i, j, km);
-    double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
-    double h12 = (uyp + u0) * (vyp + V(im, jp, k)) - (u0 + uym) * (v0 + vxm);
-    double h13 = (uzp + u0) * (wzp + W(im, j, kp)) - (u0 + uzm) * (w0 + wxm);
-    double h21 = (uxp + U(ip, jm, k)) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
-    double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
-    double h23 = (wzp + W(i, jm, kp)) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
-    double h31 = (wxp + w0) * (uxp + U(ip, j, km)) - (w0 + wxm) * (u0 + uzm);
-    double h32 = (vyp + V(i, jp, km)) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
-    double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
-#undef U
-#undef V
-#undef W
-    h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
-    h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
-    h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
-    double ru = -((h11 + h12) + h13);
-    double rv = -((h21 + h22) + h23);
-    double rw = -((h31 + h32) + h33);
-    h11 = mu * ((uxp - 2.0 * u0) + uxm) * ddxi;
-    h12 = mu * ((uyp - 2.0 * u0) + uym) * ddxi;
-    h13 = mu * ((uzp - 2.0 * u0) + uzm) * ddxi;
-    h21 = mu * ((vxp - 2.0 * v0) + vxm) * ddxi;
-    h22 = mu * ((vyp - 2.0 * v0) + vym) * ddxi;
-    h23 = mu * ((vzp - 2.0 * v0) + vzm) * ddxi;
-    h31 = mu * ((wxp - 2.0 * w0) + wxm) * ddxi;
-    h32 = mu * ((wyp - 2.0 * w0) + wym) * ddxi;
-    h33 = mu * ((wzp - 2.0 * w0) + wzm) * ddxi;
-    ru = ru + ((h11 + h12) + h13) * rhoi;
-    rv = rv + ((h21 + h22) + h23) * rhoi;
-    rw = rw + ((h31 + h32) + h33) * rhoi;
-    ru = (ru + c.f3 * sinf[k]) + c.f2 * cosf[j];
-    rv = (rv + c.f1 * sinf[i]) + c.f3 * cosf[k];
-    rw = (rw + c.f2 * sinf[j]) + c.f1 * cosf[i];
-    if (c.phiflag) {
-        // A14: CSF force at the cell centres (i,j,k), (i-1,j,k), (i,j-1,k),
-        // (i,j,k-1) from phi^{n+1}, interpolated to the faces.
-        const int imm = im == 0 ? n - 1 : im - 1, jmm = jm == 0 ? n - 1 : jm - 1, kmm = km == 0 ? n - 1 : km - 1;
-        const double sig6 = 6.0 * c.sigma, eps = c.eps, epsi = c.epsi;
-        auto chem = [&](int a, int b, int cc, int ap, int am, int bp, int bm, int cp, int cm, double f0) {
-            double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * epsi;
-            double s6 = (((((phi[IDX(ap, b, cc)] + phi[IDX(am, b, cc)]) + phi[IDX(a, bp, cc)]) + phi[IDX(a, bm, cc)]) +
-                          phi[IDX(a, b, cp)]) + phi[IDX(a, b, cm)]) - 6.0 * f0;
-            return t1 - eps * s6 * ddxi;
-        };
-        const double p0 = phi[t], pxp = phi[IDX(ip, j, k)], pxm = phi[IDX(im, j, k)];
-        const double pyp = phi[IDX(i, jp, k)], pym = phi[IDX(i, jm, k)];
-        const double pzp = phi[IDX(i, j, kp)], pzm = phi[IDX(i, j, km)];
-        const double c0 = chem(i, j, k, ip, im, jp, jm, kp, km, p0);
-        const double cxm = chem(im, j, k, i, imm, jp, jm, kp, km, pxm);
-        const double cym = chem(i, jm, k, ip, im, j, jmm, kp, km, pym);
-        const double czm = chem(i, j, km, ip, im, jp, jm, k, kmm, pzm);
-        const double fx0 = sig6 * c0 * (0.5 * (pxp - pxm) * dxi);
-        const double fy0 = sig6 * c0 * (0.5 * (pyp - pym) * dxi);
-        const double fz0 = sig6 * c0 * (0.5 * (pzp - pzm) * dxi);
-        const double fxm = sig6 * cxm * (0.5 * (p0 - phi[IDX(imm, j, k)]) * dxi);
-        const double fym = sig6 * cym * (0.5 * (p0 - phi[IDX(i, jmm, k)]) * dxi);
-        const double fzm = sig6 * czm * (0.5 * (p0 - phi[IDX(i, j, kmm)]) * dxi);
-        ru = ru + 0.5 * (fxm + fx0) * rhoi;
-        rv = rv + 0.5 * (fym + fy0) * rhoi;
-        rw = rw + 0.5 * (fzm + fz0) * rhoi;
+
+    // Box elements of this thread (<= 2 per box) and their x/y offsets.
+    size_t uoff[2], foff[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        int e = tid + q * MTHREADS;
+        uoff[q] = (size_t)wrap(i0 - 1 + e % MUX, n) + (size_t)n * wrap(j0 - 1 + e / MUX, n);
+        foff[q] = (size_t)wrap(i0 - 2 + e % MPX, n) + (size_t)n * wrap(j0 - 2 + e / MPX, n);
     }
-    const double dt = c.dt;
-    us[t] = u0 + dt * (alpha * ru - beta * ruo[t]);
-    vs[t] = v0 + dt * (alpha * rv - beta * rvo[t]);
-    ws[t] = w0 + dt * (alpha * rw - beta * rwo[t]);
-    ruo[t] = ru;
-    rvo[t] = rv;
-    rwo[t] = rw;
+    const size_t nn = (size_t)n * n;
+    auto load_u = [&](int m) {
+        const int slot = (m - k0 + 1) % 3;
+        const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            int e = tid + q * MTHREADS;
+            if (e < MUX * MUY) {
+                size_t g = uoff[q] + zoff;
+                UL[slot][0][e] = u[g] - um;
+                UL[slot][1][e] = v[g] - vm;
+                UL[slot][2][e] = w[g] - wm;
+            }
+        }
+    };
+    auto load_phi = [&](int m) {
+        if constexpr (PHI) {
+            const int slot = (m - k0 + 2) % 3;
+            const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                int e = tid + q * MTHREADS;
+                if (e < MPX * MPY) PL[slot][e] = phi[foff[q] + zoff];
+            }
+        }
+    };
+    auto force = [&](int m) {           // A14 first half on plane m
+        if constexpr (PHI) {
+            const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
+            const int fs = (m - k0 + 1) & 1;
+            const double sig6 = 6.0 * c.sigma, eps = c.eps, epsi = c.epsi;
+            for (int e = tid; e < MFX * MFY; e += MTHREADS) {
+                int fx_ = e % MFX, fy_ = e / MFX;
+                int pc = (fy_ + 1) * MPX + (fx_ + 1);
+                double f0 = PL[ps][pc];
+                double fxp = PL[ps][pc + 1], fxm = PL[ps][pc - 1];
+                double fyp = PL[ps][pc + MPX], fym = PL[ps][pc - MPX];
+                double fzp = PL[psp][pc], fzm = PL[psm][pc];
+                double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * epsi;
+                double s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - 6.0 * f0;
+                double chem = t1 - eps * s6 * ddxi;
+                FL[fs][0][e] = sig6 * chem * (0.5 * (fxp - fxm) * dxi);
+                FL[fs][1][e] = sig6 * chem * (0.5 * (fyp - fym) * dxi);
+                FL[fs][2][e] = sig6 * chem * (0.5 * (fzp - fzm) * dxi);
+            }
+        }
+    };
+
+    // prologue: u planes k0-1, k0; phi planes k0-2 .. k0; force on plane k0-1
+    load_u(k0 - 1);
+    load_u(k0);
+    load_phi(k0 - 2);
+    load_phi(k0 - 1);
+    load_phi(k0);
+    __syncthreads();
+    force(k0 - 1);
+
+    const int tx = tid % MTX, ty = tid / MTX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = (i < n) && (j < n);
+    const int kend = min(k0 + kz, n);
+    const int uc = (ty + 1) * MUX + (tx + 1);
+    const int fc = (ty + 1) * MFX + (tx + 1);
+    // Ring slots (3 each): u plane m in (m-k0+1)%3, phi plane m in (m-k0+2)%3,
+    // force plane m in (m-k0+1)%2.  phi(k+1) replaces phi(k-2), last read by
+    // force(k-1) before the previous iteration's second barrier; u(k+1)
+    // replaces u(k-2), last read by output(k-1) before this iteration's first.
+    for (int k = k0; k < kend; k++) {
+        load_phi(k + 1);
+        __syncthreads();
+        force(k);
+        load_u(k + 1);
+        __syncthreads();
+        if (active) {
+            const int s0 = (k - k0 + 1) % 3, sm = (k - k0) % 3, sp = (k - k0 + 2) % 3;
+            const double *U0 = UL[s0][0], *V0 = UL[s0][1], *W0 = UL[s0][2];
+            const double *Um = UL[sm][0], *Vm = UL[sm][1], *Wm = UL[sm][2];
+            const double *Up = UL[sp][0], *Vp = UL[sp][1], *Wp = UL[sp][2];
+            const double u0 = U0[uc], v0 = V0[uc], w0 = W0[uc];
+            const double uxp = U0[uc + 1], uxm = U0[uc - 1], uyp = U0[uc + MUX], uym = U0[uc - MUX];
+            const double uzp = Up[uc], uzm = Um[uc];
+            const double vxp = V0[uc + 1], vxm = V0[uc - 1], vyp = V0[uc + MUX], vym = V0[uc - MUX];
+            const double vzp = Vp[uc], vzm = Vm[uc];
+            const double wxp = W0[uc + 1], wxm = W0[uc - 1], wyp = W0[uc + MUX], wym = W0[uc - MUX];
+            const double wzp = Wp[uc], wzm = Wm[uc];
+            double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+            double h12 = (uyp + u0) * (vyp + V0[uc + MUX - 1]) - (u0 + uym) * (v0 + vxm);
+            double h13 = (uzp + u0) * (wzp + Wp[uc - 1]) - (u0 + uzm) * (w0 + wxm);
+            double h21 = (uxp + U0[uc + 1 - MUX]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+            double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+            double h23 = (wzp + Wp[uc - MUX]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+            double h31 = (wxp + w0) * (uxp + Um[uc + 1]) - (w0 + wxm) * (u0 + uzm);
+            double h32 = (vyp + Vm[uc + MUX]) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+            double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+            h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
+            h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
+            h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
+            double ru = -((h11 + h12) + h13);
+            double rv = -((h21 + h22) + h23);
+            double rw = -((h31 + h32) + h33);
+            h11 = mu * ((uxp - 2.0 * u0) + uxm) * ddxi;
+            h12 = mu * ((uyp - 2.0 * u0) + uym) * ddxi;
+            h13 = mu * ((uzp - 2.0 * u0) + uzm) * ddxi;
+            h21 = mu * ((vxp - 2.0 * v0) + vxm) * ddxi;
+            h22 = mu * ((vyp - 2.0 * v0) + vym) * ddxi;
+            h23 = mu * ((vzp - 2.0 * v0) + vzm) * ddxi;
+            h31 = mu * ((wxp - 2.0 * w0) + wxm) * ddxi;
+            h32 = mu * ((wyp - 2.0 * w0) + wym) * ddxi;
+            h33 = mu * ((wzp - 2.0 * w0) + wzm) * ddxi;
+            ru = ru + ((h11 + h12) + h13) * rhoi;
+            rv = rv + ((h21 + h22) + h23) * rhoi;
+            rw = rw + ((h31 + h32) + h33) * rhoi;
+            ru = (ru + c.f3 * sinf[k]) + c.f2 * cosf[j];
+            rv = (rv + c.f1 * sinf[i]) + c.f3 * cosf[k];
+            rw = (rw + c.f2 * sinf[j]) + c.f1 * cosf[i];
+            if constexpr (PHI) {
+                const int fs = (k - k0 + 1) & 1, fsm = (k - k0) & 1;
+                ru = ru + 0.5 * (FL[fs][0][fc - 1] + FL[fs][0][fc]) * rhoi;
+                rv = rv + 0.5 * (FL[fs][1][fc - MFX] + FL[fs][1][fc]) * rhoi;
+                rw = rw + 0.5 * (FL[fsm][2][fc] + FL[fs][2][fc]) * rhoi;
+            }
+            const double dt = c.dt;
+            const size_t t = IDX(i, j, k);
+            us[t] = u0 + dt * (alpha * ru - beta * ruo[t]);
+            vs[t] = v0 + dt * (alpha * rv - beta * rvo[t]);
+            ws[t] = w0 + dt * (alpha * rw - beta * rwo[t]);
+            ruo[t] = ru;
+            rvo[t] = rv;
+            rwo[t] = rw;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -609,10 +712,12 @@ void launch_debug_math(int op, size_t count, const double *x, double *y, hipStre
 
 static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
 
-static int acdi_kz(int n)
+// Planes per block for the z-marching kernels: the largest of 32..2 dividing
+// n that still gives >= 2048 blocks, else the largest of 8..2 dividing n.
+static int march_kz(int n, int tx, int ty)
 {
     for (int kz = 32; kz > 1; kz >>= 1)
-        if (n % kz == 0 && (n / kz) * ((n + ATX - 1) / ATX) * ((n + ATY - 1) / ATY) >= 2048) return kz;
+        if (n % kz == 0 && (n / kz) * ((n + tx - 1) / tx) * ((n + ty - 1) / ty) >= 2048) return kz;
     for (int kz = 8; kz > 1; kz >>= 1)
         if (n % kz == 0) return kz;
     return 1;
@@ -620,7 +725,7 @@ static int acdi_kz(int n)
 
 void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
 {
-    const int n = c.n, kz = acdi_kz(n);
+    const int n = c.n, kz = march_kz(n, ATX, ATY);
     dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, n / kz);
     hipLaunchKernelGGL(k_acdi, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, c, kz);
     li->name = "acdi"; li->bytes = 5 * F(c);
@@ -629,9 +734,14 @@ void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li
 void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta, hipStream_t s,
                      LaunchInfo *li)
 {
-    size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_momentum, dim3(nblocks(n3, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.phi, f.us,
-                       f.vs, f.ws, f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta);
+    const int n = c.n, kz = march_kz(n, MTX, MTY);
+    dim3 grid((n + MTX - 1) / MTX, (n + MTY - 1) / MTY, n / kz);
+    if (c.phiflag)
+        hipLaunchKernelGGL(k_momentum<1>, grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs, f.ws,
+                           f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
+    else
+        hipLaunchKernelGGL(k_momentum<0>, grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs, f.ws,
+                           f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
     li->name = "momentum"; li->bytes = (c.phiflag ? 13 : 12) * F(c);
 }
 

@@ -304,7 +304,7 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     const size_t n3 = (size_t)n * n * n, F = n3 * sizeof(double);
     Fields &f = c->f;
     memset(&f, 0, sizeof(f));
-    f.ld = n / 2 + 1;
+    f.ld = ((n / 2 + 1) + 7) / 8 * 8;     // 128-byte aligned rows of the half spectrum
     double **need[] = {&f.u, &f.v, &f.w, &f.us, &f.vs, &f.ws, &f.ruo, &f.rvo, &f.rwo, &f.p};
     for (double **q : need) TRY(dalloc(c, (void **)q, F));
     if (p->phiflag) {
