@@ -33,22 +33,25 @@ __device__ __forceinline__ uint3 xcd_block()
 // ---------------------------------------------------------------------------
 // Phase field, main.f90:338-486, fused into one pass (A4-A10).
 //
-// A block owns a TX x TY column of cells and marches KZ planes in z.  Two LDS
+// A block owns an ATX x ATY column of cells and marches kz planes in z.  LDS
 // rings carry what neighbouring cells share:
-//   psi (A6) on a (TX+4) x (TY+4) box for 5 planes, and
-//   the sharpening flux s = a*n (A7-A9, a = 0.25(1 - tanh^2(psi/(2 eps))), n
-//   the normalised undivided normal) on a (TX+2) x (TY+2) box for 3 planes.
-// For output plane k the kernel first adds psi of plane k+2 and s of plane
-// k+1, then forms the convective and diffusive terms (A4, A5) from global
-// memory, the sharpening divergence from the s ring, and writes
-// phi^{n+1} = phi + dt*rhs (A10) to a second buffer.  HBM traffic is one read
-// of u, v, w, phi and one write of phi^{n+1}; psi and s never leave LDS.
+//   psi (A6) on an (ATX+4) x (ATY+4) box, planes k..k+2;
+//   the sharpening flux s = a*n (A7-A9; a = 0.25(1 - tanh^2(psi/(2 eps))), n
+//   the normalised undivided normal) on an (ATX+2) x (ATY+2) box: s_x, s_y
+//   for planes k, k+1 and s_z for planes k-1..k+1;
+//   phi on the (ATX+2) x (ATY+2) box, planes k-1..k+2.
+// Iteration k adds psi(k+2) from phi values loaded one iteration earlier,
+// forms s(k+1), then the output plane k: convective and diffusive terms (A4,
+// A5), the sharpening divergence, and phi^{n+1} = phi + dt*rhs (A10) written
+// to a second buffer.  u, v, w of plane k+1 are loaded while plane k is
+// computed.  HBM traffic: u, v, w, phi read once (plus halos), phi^{n+1}
+// written once; psi and s never leave LDS.
 // ---------------------------------------------------------------------------
-constexpr int ATX = 32, ATY = 8;
+constexpr int ATX = 32, ATY = 16;
 constexpr int APX = ATX + 4, APY = ATY + 4;       // psi box
-constexpr int ASX = ATX + 2, ASY = ATY + 2;       // s box
+constexpr int ASX = ATX + 2, ASY = ATY + 2;       // s / phi box
 constexpr int ATHREADS = ATX * ATY;
-static_assert(APX * APY <= 2 * ATHREADS, "psi box loads assume <= 2 elements per thread");
+static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box elements per thread");
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
@@ -57,8 +60,10 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
     const double *__restrict__ phi, double *__restrict__ phin, const Scalars *__restrict__ sc, Consts c, int kz)
 {
     if (halted(sc)) return;
-    __shared__ double psiL[5][APY * APX];
-    __shared__ double sL[3][3][ASY * ASX];
+    __shared__ double psiL[3][APY * APX];
+    __shared__ double sxyL[2][2][ASY * ASX];
+    __shared__ double szL[3][ASY * ASX];
+    __shared__ double phL[4][ASY * ASX];
     const int n = c.n;
     const int tid = threadIdx.x;
     const uint3 blk = xcd_block();
@@ -67,75 +72,131 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
     const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const double gamma = c.gamma_factor * sc->gumax;
     const double tiny = (double)1.0E-16f;
+    const size_t nn = (size_t)n * n;
 
-    // The psi box element(s) of this thread and their x/y offsets, fixed for
-    // every plane (at most two per thread: APX*APY <= 2*ATHREADS).
+    // psi-box elements of this thread: x/y offset in the field and, when the
+    // element also lies in the inner (ATX+2) x (ATY+2) box, its index there.
     size_t poff[2];
+    int pin[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         int e = tid + q * ATHREADS;
         int bx = e % APX, by = e / APX;
         poff[q] = (size_t)wrap(i0 - 2 + bx, n) + (size_t)n * wrap(j0 - 2 + by, n);
+        bool inner = e < APX * APY && bx >= 1 && bx <= ASX && by >= 1 && by <= ASY;
+        pin[q] = inner ? (by - 1) * ASX + (bx - 1) : -1;
     }
-    const size_t nn = (size_t)n * n;
-    auto psi_plane = [&](int m) {           // A6 on plane m (any integer)
-        const int slot = (m - k0 + 2) % 5;
+    double pre[2];
+    auto fetch_phi = [&](int m) {
         const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (tid + q * ATHREADS < APX * APY) pre[q] = phi[poff[q] + zoff];
+    };
+    auto psi_plane = [&](int m) {            // A6 on plane m from pre[]
+        const int slot = (m - k0 + 3) % 3;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             int e = tid + q * ATHREADS;
             if (e < APX * APY) {
-                double f = phi[poff[q] + zoff];
+                double f = pre[q];
                 double val = f < 1.0 ? f : 1.0;
                 psiL[slot][e] = eps * dlog((val + en) / ((1.0 - val) + en));
             }
         }
     };
-    auto s_plane = [&](int m) {             // A7 + A8 + a(psi) on plane m
-        const int ps = (m - k0 + 2) % 5, psm = (m - 1 - k0 + 2) % 5, psp = (m + 1 - k0 + 2) % 5;
-        const int slot = (m - k0 + 1) % 3;
-        for (int e = tid; e < ASX * ASY; e += ATHREADS) {
-            int sx = e % ASX, sy = e / ASX;
-            int pc = (sy + 1) * APX + (sx + 1);
-            double gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
-            double gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
-            double gz = psiL[psp][pc] - psiL[psm][pc];
-            double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-            double th = dtanh((0.5 * psiL[ps][pc]) * epsi);
-            double a = 0.25 * (1.0 - th * th);
-            sL[slot][0][e] = a * (gx * normod);
-            sL[slot][1][e] = a * (gy * normod);
-            sL[slot][2][e] = a * (gz * normod);
+    auto phi_store = [&](int m) {            // phi of plane m into the inner ring
+        const int slot = (m - k0 + 4) & 3;
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (pin[q] >= 0) phL[slot][pin[q]] = pre[q];
+    };
+    auto s_plane = [&](int m) {              // A7 + A8 + a(psi) on plane m
+        const int ps = (m - k0 + 3) % 3, psm = (m - 1 - k0 + 3) % 3, psp = (m + 1 - k0 + 3) % 3;
+        const int sxy = (m - k0 + 2) & 1, sz = (m - k0 + 3) % 3;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            int e = tid + q * ATHREADS;
+            if (e < ASX * ASY) {
+                int sx = e % ASX, sy = e / ASX;
+                int pc = (sy + 1) * APX + (sx + 1);
+                double gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
+                double gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
+                double gz = psiL[psp][pc] - psiL[psm][pc];
+                double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+                double th = dtanh((0.5 * psiL[ps][pc]) * epsi);
+                double a = 0.25 * (1.0 - th * th);
+                sxyL[sxy][0][e] = a * (gx * normod);
+                sxyL[sxy][1][e] = a * (gy * normod);
+                szL[sz][e] = a * (gz * normod);
+            }
         }
     };
-
-    // prologue: psi for planes k0-2 .. k0+1, s for planes k0-1, k0
-    for (int m = k0 - 2; m <= k0 + 1; m++) psi_plane(m);
-    __syncthreads();
-    s_plane(k0 - 1);
-    s_plane(k0);
 
     const int tx = tid % ATX, ty = tid / ATX;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = (i < n) && (j < n);
-    const int ip = i + 1 >= n ? i + 1 - n : i + 1, im = i == 0 ? n - 1 : i - 1;
-    const int jp = j + 1 >= n ? j + 1 - n : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    const int ip = i + 1 >= n ? i + 1 - n : i + 1;
+    const int jp = j + 1 >= n ? j + 1 - n : j + 1;
+    const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
+    const size_t xpo = (size_t)ip + (size_t)n * (j < n ? j : 0);
+    const size_t ypo = (size_t)(i < n ? i : 0) + (size_t)n * jp;
+    double nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the next output plane
+    auto fetch_uvw = [&](int m) {
+        if (active) {
+            const size_t z = nn * (size_t)m;
+            nu0 = u[cxy + z]; nuxp = u[xpo + z];
+            nv0 = v[cxy + z]; nvyp = v[ypo + z];
+            nw0 = w[cxy + z];
+        }
+    };
+
+    // prologue: psi and phi for planes k0-2 .. k0+1, s for planes k0-1, k0.
+    // The psi ring has 3 slots, so s(k0-1) must read psi(k0-2) before
+    // psi(k0+1) takes its slot.
+    for (int m = k0 - 2; m <= k0; m++) {
+        fetch_phi(m);
+        psi_plane(m);
+        phi_store(m);
+    }
+    __syncthreads();
+    s_plane(k0 - 1);
+    fetch_phi(k0 + 1);
+    __syncthreads();
+    psi_plane(k0 + 1);
+    phi_store(k0 + 1);
+    fetch_phi(k0 + 2);
+    fetch_uvw(k0);
+    __syncthreads();
+    s_plane(k0);
+    __syncthreads();
+
     const int kend = min(k0 + kz, n);
+    const int sc0 = (ty + 1) * ASX + (tx + 1);
     for (int k = k0; k < kend; k++) {
         psi_plane(k + 2);
+        const double cu0 = nu0, cuxp = nuxp, cv0 = nv0, cvyp = nvyp, cw0 = nw0;
+        double wkp_raw = 0.0;
+        if (k + 1 < kend) {
+            fetch_uvw(k + 1);
+            wkp_raw = nw0;
+        } else if (active) {
+            wkp_raw = w[cxy + nn * (size_t)(k + 1 == n ? 0 : k + 1)];
+        }
         __syncthreads();
+        phi_store(k + 2);
+        fetch_phi(k + 3);
         s_plane(k + 1);
         __syncthreads();
         if (active) {
-            const int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
-            const size_t t = IDX(i, j, k);
-            double f0 = phi[t];
-            double fxp = phi[IDX(ip, j, k)], fxm = phi[IDX(im, j, k)];
-            double fyp = phi[IDX(i, jp, k)], fym = phi[IDX(i, jm, k)];
-            double fzp = phi[IDX(i, j, kp)], fzm = phi[IDX(i, j, km)];
-            double uip = u[IDX(ip, j, k)] - um, u0 = u[t] - um;
-            double vjp = v[IDX(i, jp, k)] - vm, v0 = v[t] - vm;
-            double wkp = w[IDX(i, j, kp)] - wm, w0 = w[t] - wm;
+            const int p0 = (k - k0 + 4) & 3, pm = (k - 1 - k0 + 4) & 3, pp = (k + 1 - k0 + 4) & 3;
+            double f0 = phL[p0][sc0];
+            double fxp = phL[p0][sc0 + 1], fxm = phL[p0][sc0 - 1];
+            double fyp = phL[p0][sc0 + ASX], fym = phL[p0][sc0 - ASX];
+            double fzp = phL[pp][sc0], fzm = phL[pm][sc0];
+            double uip = cuxp - um, u0 = cu0 - um;
+            double vjp = cvyp - vm, v0 = cv0 - vm;
+            double wkp = wkp_raw - wm, w0 = cw0 - wm;
             double A = uip * 0.5 * (fxp + f0) - u0 * 0.5 * (f0 + fxm);
             double B = vjp * 0.5 * (fyp + f0) - v0 * 0.5 * (f0 + fym);
             double C = wkp * 0.5 * (fzp + f0) - w0 * 0.5 * (f0 + fzm);
@@ -144,13 +205,12 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
             double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;
             double dzz = eps * ((fzp - 2.0 * f0) + fzm) * ddxi;
             double rphi = r + gamma * ((dxx + dyy) + dzz);
-            const int s0 = (k - k0 + 1) % 3, sm = (k - 1 - k0 + 1) % 3, sp = (k + 1 - k0 + 1) % 3;
-            const int sc0 = (ty + 1) * ASX + (tx + 1);
-            double X = (sL[s0][0][sc0 + 1] - sL[s0][0][sc0 - 1]) * 0.5 * dxi;
-            double Y = (sL[s0][1][sc0 + ASX] - sL[s0][1][sc0 - ASX]) * 0.5 * dxi;
-            double Z = (sL[sp][2][sc0] - sL[sm][2][sc0]) * 0.5 * dxi;
+            const int sxy = (k - k0 + 2) & 1, sm = (k - 1 - k0 + 3) % 3, sp = (k + 1 - k0 + 3) % 3;
+            double X = (sxyL[sxy][0][sc0 + 1] - sxyL[sxy][0][sc0 - 1]) * 0.5 * dxi;
+            double Y = (sxyL[sxy][1][sc0 + ASX] - sxyL[sxy][1][sc0 - ASX]) * 0.5 * dxi;
+            double Z = (szL[sp][sc0] - szL[sm][sc0]) * 0.5 * dxi;
             double r2 = rphi - gamma * ((X + Y) + Z);
-            phin[t] = f0 + c.dt * r2;
+            phin[cxy + nn * (size_t)k] = f0 + c.dt * r2;
         }
     }
 }
