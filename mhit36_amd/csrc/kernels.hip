@@ -262,31 +262,51 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         foff[q] = (size_t)wrap(i0 - 2 + e % MPX, n) + (size_t)n * wrap(j0 - 2 + e / MPX, n);
     }
     const size_t nn = (size_t)n * n;
-    auto load_u = [&](int m) {
-        const int slot = (m - k0 + 1) % 3;
+    // Loads run one plane ahead into registers (pu, pp) and are stored into
+    // the LDS rings an iteration later, so their latency overlaps compute.
+    double pu[2][3], pp[2];
+    auto fetch_u = [&](int m) {
         const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (tid + q * MTHREADS < MUX * MUY) {
+                size_t g = uoff[q] + zoff;
+                pu[q][0] = u[g]; pu[q][1] = v[g]; pu[q][2] = w[g];
+            }
+        }
+    };
+    auto store_u = [&](int m) {
+        const int slot = (m - k0 + 1) % 3;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             int e = tid + q * MTHREADS;
             if (e < MUX * MUY) {
-                size_t g = uoff[q] + zoff;
-                UL[slot][0][e] = u[g] - um;
-                UL[slot][1][e] = v[g] - vm;
-                UL[slot][2][e] = w[g] - wm;
+                UL[slot][0][e] = pu[q][0] - um;
+                UL[slot][1][e] = pu[q][1] - vm;
+                UL[slot][2][e] = pu[q][2] - wm;
             }
         }
     };
-    auto load_phi = [&](int m) {
+    auto fetch_phi = [&](int m) {
+        if constexpr (PHI) {
+            const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                if (tid + q * MTHREADS < MPX * MPY) pp[q] = phi[foff[q] + zoff];
+        }
+    };
+    auto store_phi = [&](int m) {
         if constexpr (PHI) {
             const int slot = (m - k0 + 2) % 3;
-            const size_t zoff = nn * (size_t)wrap(m, n);
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 int e = tid + q * MTHREADS;
-                if (e < MPX * MPY) PL[slot][e] = phi[foff[q] + zoff];
+                if (e < MPX * MPY) PL[slot][e] = pp[q];
             }
         }
     };
+    auto load_u = [&](int m) { fetch_u(m); store_u(m); };
+    auto load_phi = [&](int m) { fetch_phi(m); store_phi(m); };
     auto force = [&](int m) {           // A14 first half on plane m
         if constexpr (PHI) {
             const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
@@ -309,14 +329,18 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         }
     };
 
-    // prologue: u planes k0-1, k0; phi planes k0-2 .. k0; force on plane k0-1
+    // prologue: u planes k0-1, k0; phi planes k0-2 .. k0; force on plane k0-1;
+    // plane k0+1 fetched into registers
     load_u(k0 - 1);
     load_u(k0);
     load_phi(k0 - 2);
     load_phi(k0 - 1);
     load_phi(k0);
+    fetch_u(k0 + 1);
+    fetch_phi(k0 + 1);
     __syncthreads();
     force(k0 - 1);
+    __syncthreads();      // force(k0-1) reads phi(k0-2), whose slot phi(k0+1) takes next
 
     const int tx = tid % MTX, ty = tid / MTX;
     const int i = i0 + tx, j = j0 + ty;
@@ -324,15 +348,30 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     const int kend = min(k0 + kz, n);
     const int uc = (ty + 1) * MUX + (tx + 1);
     const int fc = (ty + 1) * MFX + (tx + 1);
+    const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
+    double ro[3] = {0.0, 0.0, 0.0};
+    if (active) {
+        ro[0] = ruo[cxy + nn * (size_t)k0];
+        ro[1] = rvo[cxy + nn * (size_t)k0];
+        ro[2] = rwo[cxy + nn * (size_t)k0];
+    }
     // Ring slots (3 each): u plane m in (m-k0+1)%3, phi plane m in (m-k0+2)%3,
     // force plane m in (m-k0+1)%2.  phi(k+1) replaces phi(k-2), last read by
     // force(k-1) before the previous iteration's second barrier; u(k+1)
     // replaces u(k-2), last read by output(k-1) before this iteration's first.
     for (int k = k0; k < kend; k++) {
-        load_phi(k + 1);
+        store_phi(k + 1);
+        if (k + 1 < kend) fetch_phi(k + 2);
         __syncthreads();
         force(k);
-        load_u(k + 1);
+        store_u(k + 1);
+        if (k + 1 < kend) fetch_u(k + 2);
+        const double ro0 = ro[0], ro1 = ro[1], ro2 = ro[2];
+        if (active && k + 1 < kend) {
+            ro[0] = ruo[cxy + nn * (size_t)(k + 1)];
+            ro[1] = rvo[cxy + nn * (size_t)(k + 1)];
+            ro[2] = rwo[cxy + nn * (size_t)(k + 1)];
+        }
         __syncthreads();
         if (active) {
             const int s0 = (k - k0 + 1) % 3, sm = (k - k0) % 3, sp = (k - k0 + 2) % 3;
@@ -383,10 +422,10 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
                 rw = rw + 0.5 * (FL[fsm][2][fc] + FL[fs][2][fc]) * rhoi;
             }
             const double dt = c.dt;
-            const size_t t = IDX(i, j, k);
-            us[t] = u0 + dt * (alpha * ru - beta * ruo[t]);
-            vs[t] = v0 + dt * (alpha * rv - beta * rvo[t]);
-            ws[t] = w0 + dt * (alpha * rw - beta * rwo[t]);
+            const size_t t = cxy + nn * (size_t)k;
+            us[t] = u0 + dt * (alpha * ru - beta * ro0);
+            vs[t] = v0 + dt * (alpha * rv - beta * ro1);
+            ws[t] = w0 + dt * (alpha * rw - beta * ro2);
             ruo[t] = ru;
             rvo[t] = rv;
             rwo[t] = rw;
