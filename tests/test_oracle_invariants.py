@@ -1,0 +1,145 @@
+"""Invariants and closed forms that pin the oracle's timestep pieces."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import orc
+
+from gpu_util import random_velocity, drops_phi
+
+
+def test_lane64_order_is_pinned():
+    """The canonical order (64 strided partial sums, then a halving tree)
+    differs from a sequential sum on this input: 2^53 + 1 rounds away in
+    lane 0, the 2.0 lands in lane 1."""
+    v = np.zeros(128)
+    v[0] = 2.0 ** 53
+    v[64] = 1.0
+    v[1] = 2.0
+    assert orc.sum_lane64(v) == 2.0 ** 53 + 2.0
+    seq = 0.0
+    for x in v:
+        seq += x
+    assert seq == 2.0 ** 53 + 4.0
+
+
+def _lane64(v):
+    acc = [0.0] * 64
+    for l in range(64):
+        for m in range(l, len(v), 64):
+            acc[l] = acc[l] + float(v[m])
+    off = 32
+    while off >= 1:
+        for l in range(off):
+            acc[l] = acc[l] + acc[l + off]
+        off //= 2
+    return acc[0]
+
+
+@pytest.mark.parametrize("n", [8, 12, 16])
+def test_sum3_restatement(n):
+    f = np.asfortranarray(np.random.default_rng(n).standard_normal((n, n, n)))
+    lines = [[_lane64(f[:, j, k]) for j in range(n)] for k in range(n)]
+    planes = [_lane64(lines[k]) for k in range(n)]
+    assert orc.sum3(f) == _lane64(planes)
+    assert abs(orc.sum3(f) - math.fsum(f.ravel())) <= 1e-13
+
+
+def test_phase_field_conserves_total_phi():
+    """A4, A5 and A9 are flux differences on a periodic grid: sum(phi) is
+    conserved to round-off by the phi update."""
+    n = 24
+    u, v, w = random_velocity(n, seed=5, amp=0.5)
+    dx = 2 * np.pi / (n - 1)
+    phi = drops_phi(n, [(2.0, 2.0, 2.0), (4.0, 4.5, 3.0)], 0.9, dx)
+    o = orc.Oracle(nx=n, dt=1e-3, mu=0.01, phiflag=1, inflow=1, inphi=1, sigma=0.1, gamma_factor=1.2)
+    for f, a in ((1, u), (2, v), (3, w), (5, phi)):
+        o.set_field(f, a)
+    s0 = math.fsum(phi.ravel())
+    o.step(3)
+    s1 = math.fsum(o.get_field(5).ravel())
+    assert abs(s1 - s0) <= 1e-12 * s0
+
+
+def test_forcing_mean_closed_form():
+    """A13 + A23: from rest, one Euler step with only f2 gives
+    u* = dt*f2*cos(k0*x(j)); with dx = 2pi/(N-1) the grid sum of cos(2 x_j)
+    is 1 (x_{N-1} = 2pi repeats x_0), so the removed mean is dt*f2/N."""
+    n, dt, f2 = 16, 1e-3, 0.7
+    o = orc.Oracle(nx=n, dt=dt, mu=0.01, f2=f2, k0=2.0, inflow=1)
+    for f in (1, 2, 3):
+        o.set_field(f, np.zeros((n, n, n)))
+    d = o.step(1)
+    assert d["umean"] == pytest.approx(dt * f2 / n, rel=1e-12)
+    assert abs(d["vmean"]) < 1e-18 and abs(d["wmean"]) < 1e-18
+
+
+def test_modified_wavenumbers_make_velocity_divergence_free():
+    """With FD2 modified wavenumbers and no REAL(4) rounding the projection
+    is exact: the discrete divergence (main.f90:723-725) of u^{n+1} is at
+    round-off, while integer wavenumbers (Q2) leave a residual."""
+    n = 16
+    u, v, w = random_velocity(n, seed=9)
+    res = {}
+    for mode in (0, 1):
+        o = orc.Oracle(nx=n, dt=1e-3, mu=0.01, f1=1.0, f2=1.0, f3=1.0, inflow=1,
+                       wavenumbers=mode, poisson_rhs_fp32=0)
+        for f, a in ((1, u), (2, v), (3, w)):
+            o.set_field(f, a)
+        o.step(2)
+        U, V, W = (o.get_field(f) for f in (1, 2, 3))
+        div = (np.roll(U, -1, 0) - U) + (np.roll(V, -1, 1) - V) + (np.roll(W, -1, 2) - W)
+        res[mode] = np.abs(div).max() / np.abs(U).max()
+    assert res[1] < 1e-13
+    assert res[0] > 1e-4
+
+
+def test_cfl_stop_and_signed_umax():
+    """main.f90:937-945: umax is the signed max; a step with cou > 7 stops."""
+    n = 16
+    o = orc.Oracle(nx=n, dt=3.0, mu=0.0)
+    o.init_ic()
+    d = o.step(5)
+    assert d["status"] == 5 and d["steps_done"] == 1
+    assert d["cou"] > 7
+    o = orc.Oracle(nx=n, dt=1e-3, mu=0.0, inflow=1)
+    for f in (1, 2, 3):
+        o.set_field(f, -np.ones((n, n, n)) * (f + 1.0))
+    d = o.step(1)
+    assert d["gumax"] == 0.0           # every component is uniform: mean removal leaves 0
+
+
+def test_nan_stops_the_run():
+    """Q7: phi below -enum makes psi NaN; it reaches u through CSF and stops."""
+    n = 16
+    o = orc.Oracle(nx=n, dt=1e-3, mu=0.01, phiflag=1, sigma=0.5)
+    o.init_ic()
+    phi = o.get_field(5)
+    phi[3, 3, 3] = -0.1
+    o.set_field(5, phi)
+    d = o.step(4)
+    assert d["status"] == 6
+
+
+def test_euler_then_ab2():
+    """main.f90:323-324, 676-678: step 1 is Euler, later steps AB2; a fresh
+    oracle from the same state repeats step 1 exactly."""
+    n = 12
+    u, v, w = random_velocity(n, seed=3)
+    a = orc.Oracle(nx=n, dt=2e-3, mu=0.02, inflow=1)
+    b = orc.Oracle(nx=n, dt=2e-3, mu=0.02, inflow=1)
+    for o in (a, b):
+        for f, x in ((1, u), (2, v), (3, w)):
+            o.set_field(f, x)
+    a.step(1)
+    b.step(1)
+    assert np.array_equal(a.get_field(1), b.get_field(1))
+    a.step(1)
+    for f in (1, 2, 3):
+        b.set_field(f, a.get_field(f) * 0 + b.get_field(f))
+    c = orc.Oracle(nx=n, dt=2e-3, mu=0.02, inflow=1)
+    for f in (1, 2, 3):
+        c.set_field(f, b.get_field(f))
+    c.step(1)                           # Euler from the state after step 1
+    assert not np.array_equal(c.get_field(1), a.get_field(1))   # AB2 differs from Euler
