@@ -29,6 +29,19 @@ def alg_bytes_per_step(n, phiflag):
     return (39.0 if phiflag else 33.0) * F
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_512cubed_ns_acdi_fp64_hbm_traffic.json")
+
+
+def measured_traffic(n, phiflag, kernel):
+    """HBM bytes per launch of `kernel` as rocprofv3 FETCH_SIZE + WRITE_SIZE
+    measured them for this workload (scaled by a 1 GiB copy; the committed
+    summary in profiles/ holds the commands), or None when not measured."""
+    if n != 512 or not phiflag or not os.path.exists(TRAFFIC_FILE):
+        return None
+    k = json.load(open(TRAFFIC_FILE))["kernels"].get(kernel)
+    return None if k is None else round(k["read_bytes"] + k["write_bytes"])
+
+
 def make_inputs(n, phiflag, seed_u=1234, seed_phi=4321, ndrops=512, radius=0.1):
     from mhit36_amd import ic
     u, v, w = ic.random_velocity(n, seed=seed_u, k_peak=4.0, urms=1.0)
@@ -143,8 +156,8 @@ def main():
         avg = dom["avg_ms"]
         ach = dom["bytes"] / (avg * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
-                "avg_ms": round(avg, 4), "alg_bytes_per_launch": dom["bytes"]}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, phiflag, dom["name"]),
+                "kernel": dom["name"], "avg_ms": round(avg, 4), "alg_bytes_per_launch": dom["bytes"]}
 
     cpu = None
     if not args.no_cpu:
