@@ -64,7 +64,10 @@ typedef struct mhit_params {
     double gamma_factor;          /* ACDI gamma = gamma_factor*gumax (1.0 = main.f90:362) */
     int wavenumbers;              /* MHIT_K_REFERENCE (integer k, main.f90:154-159) or MHIT_K_MODIFIED */
     int poisson_rhs_fp32;         /* 1: round the Poisson RHS to REAL(4) like cmplx(rhsp,0.0) (main.f90:738) */
-    int precision;                /* MHIT_FP64 (MHIT_MIXED: MHIT_ERR_UNSUPPORTED)  */
+    int precision;                /* MHIT_FP64, or MHIT_MIXED: u, v, w, u*, rhs_o and phi are
+                                     stored and advanced in FP32 (stencil RHS, forcing, phase-field
+                                     fluxes in FP32); divergence, FFT Poisson solve, projection and
+                                     the mean/umax reductions in FP64 (compensated line sums) */
     int reserved;
 } mhit_params;
 

@@ -36,17 +36,18 @@ struct Consts {
     int n;
     int phiflag;
     int fp32_rhs;
-    int pad;
+    int mixed;          // 1: u, v, w, u*, rhs_o, phi stored and advanced in FP32
     double dt, mu, rho, dxi, ddxi, rhoi, eps, epsi, enumr;
     double f1, f2, f3, sigma, gamma_factor;
 };
 
 // Device buffers of one context.
+// In mixed precision the buffers marked (T) hold FP32 values and are cast at launch.
 struct Fields {
-    double *u, *v, *w;           // velocity (stored before mean removal)
-    double *us, *vs, *ws;        // provisional velocity u*
-    double *ruo, *rvo, *rwo;     // rhs of the previous step (AB2)
-    double *phi, *phin;          // phase field, current and next
+    double *u, *v, *w;           // (T) velocity (stored before mean removal)
+    double *us, *vs, *ws;        // (T) provisional velocity u*
+    double *ruo, *rvo, *rwo;     // (T) rhs of the previous step (AB2)
+    double *phi, *phin;          // (T) phase field, current and next
     double *p;                   // pressure
     double2 *spec;               // half spectrum, [k][j][kx], leading dim ld
     double *lsum;                // per x-line partial sums, 3 * n * n

@@ -55,23 +55,31 @@ static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box e
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
+// FP64: the project's own log/tanh (bit-identical to the oracle).  FP32 (the
+// mixed-precision pass): the device libm.
+__device__ __forceinline__ double tlog(double x) { return dlog(x); }
+__device__ __forceinline__ float tlog(float x) { return logf(x); }
+__device__ __forceinline__ double ttanh(double x) { return dtanh(x); }
+__device__ __forceinline__ float ttanh(float x) { return tanhf(x); }
+
+template <typename T>
 __global__ void __launch_bounds__(ATHREADS) k_acdi(
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ phi, double *__restrict__ phin, const Scalars *__restrict__ sc, Consts c, int kz)
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, Consts c, int kz)
 {
     if (halted(sc)) return;
-    __shared__ double psiL[3][APY * APX];
-    __shared__ double sxyL[2][2][ASY * ASX];
-    __shared__ double szL[3][ASY * ASX];
-    __shared__ double phL[4][ASY * ASX];
+    __shared__ T psiL[3][APY * APX];
+    __shared__ T sxyL[2][2][ASY * ASX];
+    __shared__ T szL[3][ASY * ASX];
+    __shared__ T phL[4][ASY * ASX];
     const int n = c.n;
     const int tid = threadIdx.x;
     const uint3 blk = xcd_block();
     const int i0 = blk.x * ATX, j0 = blk.y * ATY, k0 = blk.z * kz;
-    const double eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
-    const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
-    const double gamma = c.gamma_factor * sc->gumax;
-    const double tiny = (double)1.0E-16f;
+    const T eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
+    const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    const T gamma = c.gamma_factor * sc->gumax;
+    const T tiny = (T)1.0E-16f;
     const size_t nn = (size_t)n * n;
 
     // psi-box elements of this thread: x/y offset in the field and, when the
@@ -86,7 +94,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
         bool inner = e < APX * APY && bx >= 1 && bx <= ASX && by >= 1 && by <= ASY;
         pin[q] = inner ? (by - 1) * ASX + (bx - 1) : -1;
     }
-    double pre[2];
+    T pre[2];
     auto fetch_phi = [&](int m) {
         const size_t zoff = nn * (size_t)wrap(m, n);
 #pragma unroll
@@ -99,9 +107,10 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
         for (int q = 0; q < 2; q++) {
             int e = tid + q * ATHREADS;
             if (e < APX * APY) {
-                double f = pre[q];
-                double val = f < 1.0 ? f : 1.0;
-                psiL[slot][e] = eps * dlog((val + en) / ((1.0 - val) + en));
+                T f = pre[q];
+                T val = f < (T)1.0 ? f : (T)1.0;
+                if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
+                psiL[slot][e] = eps * tlog((val + en) / (((T)1.0 - val) + en));
             }
         }
     };
@@ -120,12 +129,12 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
             if (e < ASX * ASY) {
                 int sx = e % ASX, sy = e / ASX;
                 int pc = (sy + 1) * APX + (sx + 1);
-                double gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
-                double gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
-                double gz = psiL[psp][pc] - psiL[psm][pc];
-                double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                double th = dtanh((0.5 * psiL[ps][pc]) * epsi);
-                double a = 0.25 * (1.0 - th * th);
+                T gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
+                T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
+                T gz = psiL[psp][pc] - psiL[psm][pc];
+                T normod = (T)1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+                T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi);
+                T a = (T)0.25 * ((T)1.0 - th * th);
                 sxyL[sxy][0][e] = a * (gx * normod);
                 sxyL[sxy][1][e] = a * (gy * normod);
                 szL[sz][e] = a * (gz * normod);
@@ -141,7 +150,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
     const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
     const size_t xpo = (size_t)ip + (size_t)n * (j < n ? j : 0);
     const size_t ypo = (size_t)(i < n ? i : 0) + (size_t)n * jp;
-    double nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the next output plane
+    T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the next output plane
     auto fetch_uvw = [&](int m) {
         if (active) {
             const size_t z = nn * (size_t)m;
@@ -175,8 +184,8 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
     const int sc0 = (ty + 1) * ASX + (tx + 1);
     for (int k = k0; k < kend; k++) {
         psi_plane(k + 2);
-        const double cu0 = nu0, cuxp = nuxp, cv0 = nv0, cvyp = nvyp, cw0 = nw0;
-        double wkp_raw = 0.0;
+        const T cu0 = nu0, cuxp = nuxp, cv0 = nv0, cvyp = nvyp, cw0 = nw0;
+        T wkp_raw = (T)0.0;
         if (k + 1 < kend) {
             fetch_uvw(k + 1);
             wkp_raw = nw0;
@@ -190,27 +199,27 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
         __syncthreads();
         if (active) {
             const int p0 = (k - k0 + 4) & 3, pm = (k - 1 - k0 + 4) & 3, pp = (k + 1 - k0 + 4) & 3;
-            double f0 = phL[p0][sc0];
-            double fxp = phL[p0][sc0 + 1], fxm = phL[p0][sc0 - 1];
-            double fyp = phL[p0][sc0 + ASX], fym = phL[p0][sc0 - ASX];
-            double fzp = phL[pp][sc0], fzm = phL[pm][sc0];
-            double uip = cuxp - um, u0 = cu0 - um;
-            double vjp = cvyp - vm, v0 = cv0 - vm;
-            double wkp = wkp_raw - wm, w0 = cw0 - wm;
-            double A = uip * 0.5 * (fxp + f0) - u0 * 0.5 * (f0 + fxm);
-            double B = vjp * 0.5 * (fyp + f0) - v0 * 0.5 * (f0 + fym);
-            double C = wkp * 0.5 * (fzp + f0) - w0 * 0.5 * (f0 + fzm);
-            double r = (-(A * dxi) - B * dxi) - C * dxi;
-            double dxx = eps * ((fxp - 2.0 * f0) + fxm) * ddxi;
-            double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;
-            double dzz = eps * ((fzp - 2.0 * f0) + fzm) * ddxi;
-            double rphi = r + gamma * ((dxx + dyy) + dzz);
+            T f0 = phL[p0][sc0];
+            T fxp = phL[p0][sc0 + 1], fxm = phL[p0][sc0 - 1];
+            T fyp = phL[p0][sc0 + ASX], fym = phL[p0][sc0 - ASX];
+            T fzp = phL[pp][sc0], fzm = phL[pm][sc0];
+            T uip = cuxp - um, u0 = cu0 - um;
+            T vjp = cvyp - vm, v0 = cv0 - vm;
+            T wkp = wkp_raw - wm, w0 = cw0 - wm;
+            T A = uip * (T)0.5 * (fxp + f0) - u0 * (T)0.5 * (f0 + fxm);
+            T B = vjp * (T)0.5 * (fyp + f0) - v0 * (T)0.5 * (f0 + fym);
+            T C = wkp * (T)0.5 * (fzp + f0) - w0 * (T)0.5 * (f0 + fzm);
+            T r = (-(A * dxi) - B * dxi) - C * dxi;
+            T dxx = eps * ((fxp - (T)2.0 * f0) + fxm) * ddxi;
+            T dyy = eps * ((fyp - (T)2.0 * f0) + fym) * ddxi;
+            T dzz = eps * ((fzp - (T)2.0 * f0) + fzm) * ddxi;
+            T rphi = r + gamma * ((dxx + dyy) + dzz);
             const int sxy = (k - k0 + 2) & 1, sm = (k - 1 - k0 + 3) % 3, sp = (k + 1 - k0 + 3) % 3;
-            double X = (sxyL[sxy][0][sc0 + 1] - sxyL[sxy][0][sc0 - 1]) * 0.5 * dxi;
-            double Y = (sxyL[sxy][1][sc0 + ASX] - sxyL[sxy][1][sc0 - ASX]) * 0.5 * dxi;
-            double Z = (szL[sp][sc0] - szL[sm][sc0]) * 0.5 * dxi;
-            double r2 = rphi - gamma * ((X + Y) + Z);
-            phin[cxy + nn * (size_t)k] = f0 + c.dt * r2;
+            T X = (sxyL[sxy][0][sc0 + 1] - sxyL[sxy][0][sc0 - 1]) * (T)0.5 * dxi;
+            T Y = (sxyL[sxy][1][sc0 + ASX] - sxyL[sxy][1][sc0 - ASX]) * (T)0.5 * dxi;
+            T Z = (szL[sp][sc0] - szL[sm][sc0]) * (T)0.5 * dxi;
+            T r2 = rphi - gamma * ((X + Y) + Z);
+            phin[cxy + nn * (size_t)k] = f0 + (T)c.dt * r2;
         }
     }
 }
@@ -233,25 +242,26 @@ constexpr int MPX = MTX + 3, MPY = MTY + 3;       // phi box
 constexpr int MFX = MTX + 1, MFY = MTY + 1;       // force box
 static_assert(MUX * MUY <= 2 * MTHREADS && MPX * MPY <= 2 * MTHREADS, "box loads assume <= 2 elements per thread");
 
-template <int PHI>
+template <int PHI, typename T>
 __global__ void __launch_bounds__(MTHREADS) k_momentum(
-    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
-    const double *__restrict__ phi,
-    double *__restrict__ us, double *__restrict__ vs, double *__restrict__ ws,
-    double *__restrict__ ruo, double *__restrict__ rvo, double *__restrict__ rwo,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi,
+    T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
+    T *__restrict__ ruo, T *__restrict__ rvo, T *__restrict__ rwo,
     const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
-    Consts c, double alpha, double beta, int kz)
+    Consts c, double ALPHA, double BETA, int kz)
 {
     if (halted(sc)) return;
-    __shared__ double UL[3][3][MUY * MUX];
-    __shared__ double PL[PHI ? 3 : 1][PHI ? MPY * MPX : 1];
-    __shared__ double FL[PHI ? 2 : 1][3][PHI ? MFY * MFX : 1];
+    __shared__ T UL[3][3][MUY * MUX];
+    __shared__ T PL[PHI ? 3 : 1][PHI ? MPY * MPX : 1];
+    __shared__ T FL[PHI ? 2 : 1][3][PHI ? MFY * MFX : 1];
     const int n = c.n;
     const int tid = threadIdx.x;
     const uint3 blk = xcd_block();
     const int i0 = blk.x * MTX, j0 = blk.y * MTY, k0 = blk.z * kz;
-    const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
-    const double dxi = c.dxi, ddxi = c.ddxi, mu = c.mu, rhoi = c.rhoi;
+    const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    const T dxi = (T)c.dxi, ddxi = (T)c.ddxi, mu = (T)c.mu, rhoi = (T)c.rhoi;
+    const T f1 = (T)c.f1, f2 = (T)c.f2, f3 = (T)c.f3;
 
     // Box elements of this thread (<= 2 per box) and their x/y offsets.
     size_t uoff[2], foff[2];
@@ -264,7 +274,7 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     const size_t nn = (size_t)n * n;
     // Loads run one plane ahead into registers (pu, pp) and are stored into
     // the LDS rings an iteration later, so their latency overlaps compute.
-    double pu[2][3], pp[2];
+    T pu[2][3], pp[2];
     auto fetch_u = [&](int m) {
         const size_t zoff = nn * (size_t)wrap(m, n);
 #pragma unroll
@@ -311,20 +321,20 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         if constexpr (PHI) {
             const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
             const int fs = (m - k0 + 1) & 1;
-            const double sig6 = 6.0 * c.sigma, eps = c.eps, epsi = c.epsi;
+            const T sig6 = (T)((T)6.0 * c.sigma), eps = (T)c.eps, epsi = (T)c.epsi;
             for (int e = tid; e < MFX * MFY; e += MTHREADS) {
                 int fx_ = e % MFX, fy_ = e / MFX;
                 int pc = (fy_ + 1) * MPX + (fx_ + 1);
-                double f0 = PL[ps][pc];
-                double fxp = PL[ps][pc + 1], fxm = PL[ps][pc - 1];
-                double fyp = PL[ps][pc + MPX], fym = PL[ps][pc - MPX];
-                double fzp = PL[psp][pc], fzm = PL[psm][pc];
-                double t1 = f0 * (1.0 - f0) * (1.0 - 2.0 * f0) * epsi;
-                double s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - 6.0 * f0;
-                double chem = t1 - eps * s6 * ddxi;
-                FL[fs][0][e] = sig6 * chem * (0.5 * (fxp - fxm) * dxi);
-                FL[fs][1][e] = sig6 * chem * (0.5 * (fyp - fym) * dxi);
-                FL[fs][2][e] = sig6 * chem * (0.5 * (fzp - fzm) * dxi);
+                T f0 = PL[ps][pc];
+                T fxp = PL[ps][pc + 1], fxm = PL[ps][pc - 1];
+                T fyp = PL[ps][pc + MPX], fym = PL[ps][pc - MPX];
+                T fzp = PL[psp][pc], fzm = PL[psm][pc];
+                T t1 = f0 * ((T)1.0 - f0) * ((T)1.0 - (T)2.0 * f0) * epsi;
+                T s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - (T)6.0 * f0;
+                T chem = t1 - eps * s6 * ddxi;
+                FL[fs][0][e] = sig6 * chem * ((T)0.5 * (fxp - fxm) * dxi);
+                FL[fs][1][e] = sig6 * chem * ((T)0.5 * (fyp - fym) * dxi);
+                FL[fs][2][e] = sig6 * chem * ((T)0.5 * (fzp - fzm) * dxi);
             }
         }
     };
@@ -349,7 +359,7 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     const int uc = (ty + 1) * MUX + (tx + 1);
     const int fc = (ty + 1) * MFX + (tx + 1);
     const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
-    double ro[3] = {0.0, 0.0, 0.0};
+    T ro[3] = {(T)0.0, (T)0.0, (T)0.0};
     if (active) {
         ro[0] = ruo[cxy + nn * (size_t)k0];
         ro[1] = rvo[cxy + nn * (size_t)k0];
@@ -366,7 +376,7 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         force(k);
         store_u(k + 1);
         if (k + 1 < kend) fetch_u(k + 2);
-        const double ro0 = ro[0], ro1 = ro[1], ro2 = ro[2];
+        const T ro0 = ro[0], ro1 = ro[1], ro2 = ro[2];
         if (active && k + 1 < kend) {
             ro[0] = ruo[cxy + nn * (size_t)(k + 1)];
             ro[1] = rvo[cxy + nn * (size_t)(k + 1)];
@@ -375,53 +385,53 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         __syncthreads();
         if (active) {
             const int s0 = (k - k0 + 1) % 3, sm = (k - k0) % 3, sp = (k - k0 + 2) % 3;
-            const double *U0 = UL[s0][0], *V0 = UL[s0][1], *W0 = UL[s0][2];
-            const double *Um = UL[sm][0], *Vm = UL[sm][1], *Wm = UL[sm][2];
-            const double *Up = UL[sp][0], *Vp = UL[sp][1], *Wp = UL[sp][2];
-            const double u0 = U0[uc], v0 = V0[uc], w0 = W0[uc];
-            const double uxp = U0[uc + 1], uxm = U0[uc - 1], uyp = U0[uc + MUX], uym = U0[uc - MUX];
-            const double uzp = Up[uc], uzm = Um[uc];
-            const double vxp = V0[uc + 1], vxm = V0[uc - 1], vyp = V0[uc + MUX], vym = V0[uc - MUX];
-            const double vzp = Vp[uc], vzm = Vm[uc];
-            const double wxp = W0[uc + 1], wxm = W0[uc - 1], wyp = W0[uc + MUX], wym = W0[uc - MUX];
-            const double wzp = Wp[uc], wzm = Wm[uc];
-            double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
-            double h12 = (uyp + u0) * (vyp + V0[uc + MUX - 1]) - (u0 + uym) * (v0 + vxm);
-            double h13 = (uzp + u0) * (wzp + Wp[uc - 1]) - (u0 + uzm) * (w0 + wxm);
-            double h21 = (uxp + U0[uc + 1 - MUX]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
-            double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
-            double h23 = (wzp + Wp[uc - MUX]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
-            double h31 = (wxp + w0) * (uxp + Um[uc + 1]) - (w0 + wxm) * (u0 + uzm);
-            double h32 = (vyp + Vm[uc + MUX]) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
-            double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
-            h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
-            h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
-            h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
-            double ru = -((h11 + h12) + h13);
-            double rv = -((h21 + h22) + h23);
-            double rw = -((h31 + h32) + h33);
-            h11 = mu * ((uxp - 2.0 * u0) + uxm) * ddxi;
-            h12 = mu * ((uyp - 2.0 * u0) + uym) * ddxi;
-            h13 = mu * ((uzp - 2.0 * u0) + uzm) * ddxi;
-            h21 = mu * ((vxp - 2.0 * v0) + vxm) * ddxi;
-            h22 = mu * ((vyp - 2.0 * v0) + vym) * ddxi;
-            h23 = mu * ((vzp - 2.0 * v0) + vzm) * ddxi;
-            h31 = mu * ((wxp - 2.0 * w0) + wxm) * ddxi;
-            h32 = mu * ((wyp - 2.0 * w0) + wym) * ddxi;
-            h33 = mu * ((wzp - 2.0 * w0) + wzm) * ddxi;
+            const T *U0 = UL[s0][0], *V0 = UL[s0][1], *W0 = UL[s0][2];
+            const T *Um = UL[sm][0], *Vm = UL[sm][1], *Wm = UL[sm][2];
+            const T *Up = UL[sp][0], *Vp = UL[sp][1], *Wp = UL[sp][2];
+            const T u0 = U0[uc], v0 = V0[uc], w0 = W0[uc];
+            const T uxp = U0[uc + 1], uxm = U0[uc - 1], uyp = U0[uc + MUX], uym = U0[uc - MUX];
+            const T uzp = Up[uc], uzm = Um[uc];
+            const T vxp = V0[uc + 1], vxm = V0[uc - 1], vyp = V0[uc + MUX], vym = V0[uc - MUX];
+            const T vzp = Vp[uc], vzm = Vm[uc];
+            const T wxp = W0[uc + 1], wxm = W0[uc - 1], wyp = W0[uc + MUX], wym = W0[uc - MUX];
+            const T wzp = Wp[uc], wzm = Wm[uc];
+            T h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+            T h12 = (uyp + u0) * (vyp + V0[uc + MUX - 1]) - (u0 + uym) * (v0 + vxm);
+            T h13 = (uzp + u0) * (wzp + Wp[uc - 1]) - (u0 + uzm) * (w0 + wxm);
+            T h21 = (uxp + U0[uc + 1 - MUX]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+            T h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+            T h23 = (wzp + Wp[uc - MUX]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+            T h31 = (wxp + w0) * (uxp + Um[uc + 1]) - (w0 + wxm) * (u0 + uzm);
+            T h32 = (vyp + Vm[uc + MUX]) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+            T h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+            h11 = h11 * (T)0.25 * dxi; h12 = h12 * (T)0.25 * dxi; h13 = h13 * (T)0.25 * dxi;
+            h21 = h21 * (T)0.25 * dxi; h22 = h22 * (T)0.25 * dxi; h23 = h23 * (T)0.25 * dxi;
+            h31 = h31 * (T)0.25 * dxi; h32 = h32 * (T)0.25 * dxi; h33 = h33 * (T)0.25 * dxi;
+            T ru = -((h11 + h12) + h13);
+            T rv = -((h21 + h22) + h23);
+            T rw = -((h31 + h32) + h33);
+            h11 = mu * ((uxp - (T)2.0 * u0) + uxm) * ddxi;
+            h12 = mu * ((uyp - (T)2.0 * u0) + uym) * ddxi;
+            h13 = mu * ((uzp - (T)2.0 * u0) + uzm) * ddxi;
+            h21 = mu * ((vxp - (T)2.0 * v0) + vxm) * ddxi;
+            h22 = mu * ((vyp - (T)2.0 * v0) + vym) * ddxi;
+            h23 = mu * ((vzp - (T)2.0 * v0) + vzm) * ddxi;
+            h31 = mu * ((wxp - (T)2.0 * w0) + wxm) * ddxi;
+            h32 = mu * ((wyp - (T)2.0 * w0) + wym) * ddxi;
+            h33 = mu * ((wzp - (T)2.0 * w0) + wzm) * ddxi;
             ru = ru + ((h11 + h12) + h13) * rhoi;
             rv = rv + ((h21 + h22) + h23) * rhoi;
             rw = rw + ((h31 + h32) + h33) * rhoi;
-            ru = (ru + c.f3 * sinf[k]) + c.f2 * cosf[j];
-            rv = (rv + c.f1 * sinf[i]) + c.f3 * cosf[k];
-            rw = (rw + c.f2 * sinf[j]) + c.f1 * cosf[i];
+            ru = (ru + f3 * (T)sinf[k]) + f2 * (T)cosf[j];
+            rv = (rv + f1 * (T)sinf[i]) + f3 * (T)cosf[k];
+            rw = (rw + f2 * (T)sinf[j]) + f1 * (T)cosf[i];
             if constexpr (PHI) {
                 const int fs = (k - k0 + 1) & 1, fsm = (k - k0) & 1;
-                ru = ru + 0.5 * (FL[fs][0][fc - 1] + FL[fs][0][fc]) * rhoi;
-                rv = rv + 0.5 * (FL[fs][1][fc - MFX] + FL[fs][1][fc]) * rhoi;
-                rw = rw + 0.5 * (FL[fsm][2][fc] + FL[fs][2][fc]) * rhoi;
+                ru = ru + (T)0.5 * (FL[fs][0][fc - 1] + FL[fs][0][fc]) * rhoi;
+                rv = rv + (T)0.5 * (FL[fs][1][fc - MFX] + FL[fs][1][fc]) * rhoi;
+                rw = rw + (T)0.5 * (FL[fsm][2][fc] + FL[fs][2][fc]) * rhoi;
             }
-            const double dt = c.dt;
+            const T dt = (T)c.dt, alpha = (T)ALPHA, beta = (T)BETA;
             const size_t t = cxy + nn * (size_t)k;
             us[t] = u0 + dt * (alpha * ru - beta * ro0);
             vs[t] = v0 + dt * (alpha * rv - beta * ro1);
@@ -501,8 +511,9 @@ __device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, cons
 // A16 + A17 + forward x pass: divergence of u*, REAL(4) rounding of the
 // Poisson RHS, then one complex FFT per pair of x-lines (j, j+1) whose
 // spectrum is split into the two half spectra (main.f90:715-750).
+template <typename T>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
-    const double *__restrict__ us, const double *__restrict__ vs, const double *__restrict__ ws,
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
     Consts c, FftPlan pl, int ppb, int ld)
 {
@@ -524,9 +535,9 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
             int jj = j + h;
             int jp = jj + 1 == n ? 0 : jj + 1;
             size_t id = IDX(i, jj, k);
-            double r = cdiv * (us[IDX(ip, jj, k)] - us[id]);
-            r = r + cdiv * (vs[IDX(i, jp, k)] - vs[id]);
-            r = r + cdiv * (ws[IDX(i, jj, kp)] - ws[id]);
+            double r = cdiv * ((double)us[IDX(ip, jj, k)] - (double)us[id]);
+            r = r + cdiv * ((double)vs[IDX(i, jp, k)] - (double)vs[id]);
+            r = r + cdiv * ((double)ws[IDX(i, jj, kp)] - (double)ws[id]);
             if (c.fp32_rhs) r = (double)(float)r;
             rr[h] = r;
         }
@@ -670,6 +681,15 @@ __device__ __forceinline__ double nanmax(double a, double b)
     return a > b ? a : b;
 }
 
+// Neumaier's compensated summation step (s, c) += x.
+__device__ __forceinline__ void neumaier(double &s, double &c, double x)
+{
+    double t = s + x;
+    if (__builtin_fabs(s) >= __builtin_fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
 __device__ __forceinline__ double wave_sum(double x)
 {
 #pragma unroll
@@ -684,9 +704,10 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(BLK) k_project(
-    const double *__restrict__ us, const double *__restrict__ vs, const double *__restrict__ ws,
-    const double *__restrict__ p, double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+    const double *__restrict__ p, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
     double *__restrict__ lsum, double *__restrict__ lmax, const Scalars *__restrict__ sc, Consts c)
 {
     if (halted(sc)) return;
@@ -698,20 +719,27 @@ __global__ void __launch_bounds__(BLK) k_project(
     const int jm = j == 0 ? n - 1 : j - 1, km = k == 0 ? n - 1 : k - 1;
     const double dtr = c.dt / c.rho, dxi = c.dxi;
     double su = 0.0, sv = 0.0, sw = 0.0;
+    double cu = 0.0, cv = 0.0, cw = 0.0;     // Neumaier compensation (FP32 storage only)
     double mu_ = -__builtin_inf(), mv = -__builtin_inf(), mw = -__builtin_inf();
     for (int i = lane; i < n; i += 64) {
         int im = i == 0 ? n - 1 : i - 1;
         size_t id = IDX(i, j, k);
         double p0 = p[id];
-        double uu = us[id] - dtr * (p0 - p[IDX(im, j, k)]) * dxi;
-        double vv = vs[id] - dtr * (p0 - p[IDX(i, jm, k)]) * dxi;
-        double ww = ws[id] - dtr * (p0 - p[IDX(i, j, km)]) * dxi;
-        u[id] = uu;
-        v[id] = vv;
-        w[id] = ww;
-        su = su + uu; sv = sv + vv; sw = sw + ww;
+        const T tu = (T)((double)us[id] - dtr * (p0 - p[IDX(im, j, k)]) * dxi);
+        const T tv = (T)((double)vs[id] - dtr * (p0 - p[IDX(i, jm, k)]) * dxi);
+        const T tw = (T)((double)ws[id] - dtr * (p0 - p[IDX(i, j, km)]) * dxi);
+        u[id] = tu;
+        v[id] = tv;
+        w[id] = tw;
+        const double uu = tu, vv = tv, ww = tw;
+        if constexpr (sizeof(T) == 4) {
+            neumaier(su, cu, uu); neumaier(sv, cv, vv); neumaier(sw, cw, ww);
+        } else {
+            su = su + uu; sv = sv + vv; sw = sw + ww;
+        }
         mu_ = nanmax(mu_, uu); mv = nanmax(mv, vv); mw = nanmax(mw, ww);
     }
+    if constexpr (sizeof(T) == 4) { su = su + cu; sv = sv + cv; sw = sw + cw; }
     su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
     mu_ = wave_max(mu_); mv = wave_max(mv); mw = wave_max(mw);
     if (lane == 0) {
@@ -826,8 +854,12 @@ void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li
 {
     const int n = c.n, kz = march_kz(n, ATX, ATY);
     dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, n / kz);
-    hipLaunchKernelGGL(k_acdi, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, c, kz);
-    li->name = "acdi"; li->bytes = 5 * F(c);
+    if (c.mixed)
+        hipLaunchKernelGGL(k_acdi<float>, grid, dim3(ATHREADS), 0, s, (const float *)f.u, (const float *)f.v,
+                           (const float *)f.w, (const float *)f.phi, (float *)f.phin, f.sc, c, kz);
+    else
+        hipLaunchKernelGGL(k_acdi<double>, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, c, kz);
+    li->name = "acdi"; li->bytes = (c.mixed ? 2.5 : 5.0) * F(c);
 }
 
 void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta, hipStream_t s,
@@ -835,13 +867,25 @@ void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta
 {
     const int n = c.n, kz = march_kz(n, MTX, MTY);
     dim3 grid((n + MTX - 1) / MTX, (n + MTY - 1) / MTY, n / kz);
-    if (c.phiflag)
-        hipLaunchKernelGGL(k_momentum<1>, grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs, f.ws,
-                           f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
-    else
-        hipLaunchKernelGGL(k_momentum<0>, grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs, f.ws,
-                           f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
-    li->name = "momentum"; li->bytes = (c.phiflag ? 13 : 12) * F(c);
+    if (c.mixed) {
+        using T = float;
+        const T *u = (const T *)f.u, *v = (const T *)f.v, *w = (const T *)f.w, *ph = (const T *)f.phi;
+        T *us = (T *)f.us, *vs = (T *)f.vs, *ws = (T *)f.ws, *ru = (T *)f.ruo, *rv = (T *)f.rvo, *rw = (T *)f.rwo;
+        if (c.phiflag)
+            hipLaunchKernelGGL((k_momentum<1, T>), grid, dim3(MTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+                               f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
+        else
+            hipLaunchKernelGGL((k_momentum<0, T>), grid, dim3(MTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+                               f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
+    } else {
+        if (c.phiflag)
+            hipLaunchKernelGGL((k_momentum<1, double>), grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs,
+                               f.ws, f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
+        else
+            hipLaunchKernelGGL((k_momentum<0, double>), grid, dim3(MTHREADS), 0, s, f.u, f.v, f.w, f.phi, f.us, f.vs,
+                               f.ws, f.ruo, f.rvo, f.rwo, f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
+    }
+    li->name = "momentum"; li->bytes = (c.mixed ? (c.phiflag ? 6.5 : 6.0) : (c.phiflag ? 13.0 : 12.0)) * F(c);
 }
 
 static int plan_rmin(const FftPlan &pl)
@@ -886,9 +930,13 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
     int ppb = pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
-    hipLaunchKernelGGL(k_xfwd, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, f.us, f.vs, f.ws, f.spec,
-                       f.tw, f.sc, c, pl, ppb, f.ld);
-    li->name = "xfwd_div"; li->bytes = 4 * F(c);
+    if (c.mixed)
+        hipLaunchKernelGGL(k_xfwd<float>, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, (const float *)f.us,
+                           (const float *)f.vs, (const float *)f.ws, f.spec, f.tw, f.sc, c, pl, ppb, f.ld);
+    else
+        hipLaunchKernelGGL(k_xfwd<double>, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, f.us, f.vs, f.ws,
+                           f.spec, f.tw, f.sc, c, pl, ppb, f.ld);
+    li->name = "xfwd_div"; li->bytes = (c.mixed ? 2.5 : 4.0) * F(c);
 }
 
 void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse, hipStream_t s,
@@ -932,9 +980,14 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
 {
     size_t threads = (size_t)c.n * c.n * 64;
-    hipLaunchKernelGGL(k_project, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, f.us, f.vs, f.ws, f.p, f.u, f.v,
-                       f.w, f.lsum, f.lmax, f.sc, c);
-    li->name = "project"; li->bytes = 7 * F(c);
+    if (c.mixed)
+        hipLaunchKernelGGL(k_project<float>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, (const float *)f.us,
+                           (const float *)f.vs, (const float *)f.ws, f.p, (float *)f.u, (float *)f.v, (float *)f.w,
+                           f.lsum, f.lmax, f.sc, c);
+    else
+        hipLaunchKernelGGL(k_project<double>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, f.us, f.vs, f.ws, f.p,
+                           f.u, f.v, f.w, f.lsum, f.lmax, f.sc, c);
+    li->name = "project"; li->bytes = (c.mixed ? 4.0 : 7.0) * F(c);
 }
 
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)

@@ -249,7 +249,7 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     const int n = p->nx;
     int rad[MAXSTAGES + 4];
     if (n < 4 || n % 2 != 0 || plan_radices(n, rad) == 0 || n > 2048) return MHIT_ERR_BAD_N;
-    if (p->precision != MHIT_FP64) return MHIT_ERR_UNSUPPORTED;
+    if (p->precision != MHIT_FP64 && p->precision != MHIT_MIXED) return MHIT_ERR_UNSUPPORTED;
     if (p->dt == 0.0 || p->rho == 0.0 || p->epsr == 0.0) return MHIT_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -280,6 +280,7 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     k.n = n;
     k.phiflag = p->phiflag ? 1 : 0;
     k.fp32_rhs = p->poisson_rhs_fp32 ? 1 : 0;
+    k.mixed = p->precision == MHIT_MIXED ? 1 : 0;
     k.dt = p->dt; k.mu = p->mu; k.rho = p->rho;
     k.dxi = c->cs.dxi; k.ddxi = c->cs.ddxi; k.rhoi = c->cs.rhoi;
     k.eps = c->cs.eps; k.epsi = c->cs.epsi; k.enumr = c->cs.enum_;
@@ -302,14 +303,16 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     if (plan_build(n, &c->plan, &tw) != 0) { mhit_destroy(c); return MHIT_ERR_BAD_N; }
 
     const size_t n3 = (size_t)n * n * n, F = n3 * sizeof(double);
+    const size_t FT = n3 * (k.mixed ? sizeof(float) : sizeof(double));   // state fields
     Fields &f = c->f;
     memset(&f, 0, sizeof(f));
     f.ld = ((n / 2 + 1) + 7) / 8 * 8;     // 128-byte aligned rows of the half spectrum
-    double **need[] = {&f.u, &f.v, &f.w, &f.us, &f.vs, &f.ws, &f.ruo, &f.rvo, &f.rwo, &f.p};
-    for (double **q : need) TRY(dalloc(c, (void **)q, F));
+    double **need[] = {&f.u, &f.v, &f.w, &f.us, &f.vs, &f.ws, &f.ruo, &f.rvo, &f.rwo};
+    for (double **q : need) TRY(dalloc(c, (void **)q, FT));
+    TRY(dalloc(c, (void **)&f.p, F));
     if (p->phiflag) {
         double **ph[] = {&f.phi, &f.phin};
-        for (double **q : ph) TRY(dalloc(c, (void **)q, F));
+        for (double **q : ph) TRY(dalloc(c, (void **)q, FT));
     }
     TRY(dalloc(c, (void **)&f.spec, sizeof(double2) * (size_t)n * n * f.ld));
     TRY(dalloc(c, (void **)&f.lsum, sizeof(double) * 3 * (size_t)n * n));
@@ -366,7 +369,13 @@ extern "C" int mhit_set_field(mhit_ctx *c, int fieldn, const double *host)
     if (!d) return MHIT_ERR_ARG;
     const size_t n = c->p.nx, n3 = n * n * n;
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(d, host, n3 * sizeof(double), hipMemcpyHostToDevice));
+    if (c->kc.mixed && fieldn != 4) {
+        std::vector<float> tmp(n3);
+        for (size_t q = 0; q < n3; q++) tmp[q] = (float)host[q];
+        HIPCHK(hipMemcpy(d, tmp.data(), n3 * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(d, host, n3 * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (fieldn <= 3) {
         // the stored field is now the field itself: its pending mean is zero
         double zero = 0.0;
@@ -385,7 +394,13 @@ extern "C" int mhit_get_field(mhit_ctx *c, int fieldn, double *host)
     Scalars s;
     int rc = read_scalars(c, &s);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(host, d, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (c->kc.mixed && fieldn != 4) {
+        std::vector<float> tmp(n3);
+        HIPCHK(hipMemcpy(tmp.data(), d, n3 * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n3; q++) host[q] = (double)tmp[q];
+    } else {
+        HIPCHK(hipMemcpy(host, d, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    }
     if (fieldn <= 3) {
         // u - gumean (main.f90:917-919), the rounding every kernel applies
         double m = fieldn == 1 ? s.umean : (fieldn == 2 ? s.vmean : s.wmean);
@@ -661,6 +676,7 @@ extern "C" int mhit_debug_buffer(mhit_ctx *c, int which, double *host)
     default: return MHIT_ERR_ARG;
     }
     if (!d) return MHIT_ERR_ARG;
+    if (c->kc.mixed) return MHIT_ERR_UNSUPPORTED;
     const size_t n = c->p.nx, n3 = n * n * n;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(host, d, n3 * sizeof(double), hipMemcpyDeviceToHost));

@@ -92,7 +92,7 @@ def test_create_rejects_bad_grid_before_touching_the_device():
     for n in (0, 2, 7, 10, 50, 4096):
         p = default_params(nx=n)
         assert lib.mhit_create(ctypes.byref(p), ctypes.byref(h)) == _lib.MHIT_ERR_BAD_N
-    p = default_params(nx=32, precision=1)
+    p = default_params(nx=32, precision=2)
     assert lib.mhit_create(ctypes.byref(p), ctypes.byref(h)) == _lib.MHIT_ERR_UNSUPPORTED
     p = default_params(nx=32, dt=0.0)
     assert lib.mhit_create(ctypes.byref(p), ctypes.byref(h)) == _lib.MHIT_ERR_ARG
