@@ -669,6 +669,243 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restr
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident variant of the same Stockham FFT, for plans whose radices
+// are 8, 4 and 2.  A line of length n is owned by LM = n/8 threads; thread tj
+// holds the 8 elements at positions tj + m*LM (m = 0..7), which are exactly
+// the inputs of its radix-8 butterfly (or of its 8/R radix-R butterflies, the
+// q-th using m = q + r*8/R), and also exactly the outputs of the last stage.
+// Hence the first stage reads straight from global memory, the last stage
+// writes straight to global memory (or feeds the inverse transform), and LDS
+// only carries the exchanges between stages.  Every butterfly, twiddle and
+// operation order is the one of fft_stage, so results are bit-identical.
+// ---------------------------------------------------------------------------
+template <bool XL>
+__device__ __forceinline__ int rlds(int line, int p, int lstride, int pstride)
+{
+    return XL ? line * lstride + p + (p >> 3) : line + p * pstride;
+}
+
+template <int R, int SG>
+__device__ __forceinline__ void stage_regs(double2 (&v)[8], int tj, int LM, int Ns, const double2 *__restrict__ tw,
+                                           bool first)
+{
+    constexpr int Q = 8 / R;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int jj = tj + q * LM;
+        const int k = jj % Ns;
+        double2 x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = v[q + r * Q];
+        if (!first) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                double2 wv = tw[k * (R - 1) + r - 1];
+                double wi = SG < 0 ? -wv.y : wv.y;
+                double xr = x[r].x, xi = x[r].y;
+                x[r] = make_double2(xr * wv.x - xi * wi, xr * wi + xi * wv.x);
+            }
+        }
+        dft<R, SG>(x);
+#pragma unroll
+        for (int r = 0; r < R; r++) v[q + r * Q] = x[r];
+    }
+}
+
+template <int R, bool XL>
+__device__ __forceinline__ void store_stage(const double2 (&v)[8], double2 *lds, int line, int tj, int LM, int Ns,
+                                            int lstride, int pstride)
+{
+    constexpr int Q = 8 / R;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int jj = tj + q * LM;
+        const int base = (jj / Ns) * Ns * R + jj % Ns;
+#pragma unroll
+        for (int r = 0; r < R; r++) lds[rlds<XL>(line, base + r * Ns, lstride, pstride)] = v[q + r * Q];
+    }
+}
+
+// All stages of one line; v holds positions tj + m*LM on entry and on exit.
+template <int SG, bool XL>
+__device__ void fft_regs(double2 (&v)[8], int tj, int line, double2 *lds, int lstride, int pstride,
+                         const FftPlan &pl, const double2 *__restrict__ tw)
+{
+    const int LM = pl.n / 8;
+    for (int s = 0; s < pl.nst; s++) {
+        if (s > 0) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) v[m] = lds[rlds<XL>(line, tj + m * LM, lstride, pstride)];
+            __syncthreads();
+        }
+        const double2 *t = tw + pl.twoff[s];
+        const int R = pl.R[s], Ns = pl.Ns[s];
+        if (R == 8) stage_regs<8, SG>(v, tj, LM, Ns, t, s == 0);
+        else if (R == 4) stage_regs<4, SG>(v, tj, LM, Ns, t, s == 0);
+        else stage_regs<2, SG>(v, tj, LM, Ns, t, s == 0);
+        if (s < pl.nst - 1) {
+            if (R == 8) store_stage<8, XL>(v, lds, line, tj, LM, Ns, lstride, pstride);
+            else if (R == 4) store_stage<4, XL>(v, lds, line, tj, LM, Ns, lstride, pstride);
+            else store_stage<2, XL>(v, lds, line, tj, LM, Ns, lstride, pstride);
+            __syncthreads();
+        }
+    }
+}
+
+__host__ __device__ constexpr int rline_stride(int n) { return n + n / 8 + 1; }
+
+template <typename T>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+    double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
+    Consts c, FftPlan pl, int ppb, int ld)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = c.n, half = n / 2, nh = n / 2 + 1, LM = n / 8;
+    const int lstride = rline_stride(n);
+    const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
+    const int pair = blockIdx.x * ppb + pl_;
+    const int k = pair / half, j = 2 * (pair - k * half);
+    const int kp = k + 1 == n ? 0 : k + 1;
+    const double cdiv = c.rho * c.dxi / c.dt;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int i = tj + m * LM;
+        const int ip = i + 1 == n ? 0 : i + 1;
+        double rr[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int jj = j + h;
+            int jp = jj + 1 == n ? 0 : jj + 1;
+            size_t id = IDX(i, jj, k);
+            double r = cdiv * ((double)us[IDX(ip, jj, k)] - (double)us[id]);
+            r = r + cdiv * ((double)vs[IDX(i, jp, k)] - (double)vs[id]);
+            r = r + cdiv * ((double)ws[IDX(i, jj, kp)] - (double)ws[id]);
+            if (c.fp32_rhs) r = (double)(float)r;
+            rr[h] = r;
+        }
+        v[m] = make_double2(rr[0], rr[1]);
+    }
+    fft_regs<-1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
+#pragma unroll
+    for (int m = 0; m < 8; m++) lds[rlds<true>(pl_, tj + m * LM, lstride, 1)] = v[m];
+    __syncthreads();
+    for (int e = t; e < ppb * nh; e += blockDim.x) {
+        int pq = e / nh, q = e - pq * nh;
+        int pr = blockIdx.x * ppb + pq;
+        int kk = pr / half, jj = 2 * (pr - kk * half);
+        int mq = q == 0 ? 0 : n - q;
+        double2 zq = lds[rlds<true>(pq, q, lstride, 1)], zm = lds[rlds<true>(pq, mq, lstride, 1)];
+        double a = zq.x, b = zq.y, cc = zm.x, d = zm.y;
+        spec[((size_t)kk * n + jj) * ld + q] = make_double2((a + cc) * 0.5, (b - d) * 0.5);
+        spec[((size_t)kk * n + jj + 1) * ld + q] = make_double2((b + d) * 0.5, (cc - a) * 0.5);
+    }
+}
+
+template <int SG>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict__ spec,
+                                                             const double2 *__restrict__ tw,
+                                                             const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                             int tkx)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
+    const int q0 = blockIdx.x * tkx, k = blockIdx.y;
+    const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
+    const bool ok = q0 + line < nh;
+    double2 *base = spec + (size_t)k * n * ld + q0 + line;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * ld] : make_double2(0.0, 0.0);
+    fft_regs<SG, false>(v, tj, line, lds, 1, tkx, pl, tw);
+    if (ok) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) base[(size_t)(tj + m * LM) * ld] = v[m];
+    }
+}
+
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restrict__ spec,
+                                                                const double2 *__restrict__ tw,
+                                                                const double *__restrict__ ktab,
+                                                                const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                                int tkx)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
+    const int q0 = blockIdx.x * tkx, j = blockIdx.y;
+    const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
+    const int q = q0 + line;
+    const bool ok = q < nh;
+    const size_t rs = (size_t)n * ld;
+    double2 *base = spec + (size_t)j * ld + q;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * rs] : make_double2(0.0, 0.0);
+    fft_regs<-1, false>(v, tj, line, lds, 1, tkx, pl, tw);
+    const double nnn = (double)((long long)n * (long long)n * (long long)n);
+    const double ky = ktab[j], kx = ktab[ok ? q : 0];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int r = tj + m * LM;
+        double kz = ktab[r];
+        double k2 = (kx * kx + ky * ky) + kz * kz;
+        double2 z = v[m];
+        z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
+        if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
+        v[m] = z;
+    }
+    fft_regs<1, false>(v, tj, line, lds, 1, tkx, pl, tw);
+    if (ok) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) base[(size_t)(tj + m * LM) * rs] = v[m];
+    }
+}
+
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__restrict__ spec,
+                                                            double *__restrict__ p, const double2 *__restrict__ tw,
+                                                            const Scalars *__restrict__ sc, FftPlan pl, int ppb,
+                                                            int ld)
+{
+    if (halted(sc)) return;
+    extern __shared__ double2 lds[];
+    const int n = pl.n, half = n / 2, LM = n / 8;
+    const int lstride = rline_stride(n);
+    const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
+    const int pair = blockIdx.x * ppb + pl_;
+    const int k = pair / half, j = 2 * (pair - k * half);
+    const double2 *s1 = spec + ((size_t)k * n + j) * ld;
+    const double2 *s2 = spec + ((size_t)k * n + j + 1) * ld;
+    double2 v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int q = tj + m * LM;
+        double2 z;
+        if (q == 0 || q == half) {
+            z = make_double2(s1[q].x, s2[q].x);
+        } else if (q < half) {
+            double2 a = s1[q], b = s2[q];
+            z = make_double2(a.x - b.y, a.y + b.x);
+        } else {
+            int mm = n - q;
+            double2 a = s1[mm], b = s2[mm];
+            z = make_double2(a.x + b.y, b.x - a.y);
+        }
+        v[m] = z;
+    }
+    fft_regs<1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int i = tj + m * LM;
+        p[IDX(i, j, k)] = v[m].x;
+        p[IDX(i, j + 1, k)] = v[m].y;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // A22 projection (main.f90:873-887) with the per-x-line partial sums and
 // maxima behind A23/A25.  One 64-wide wavefront per x-line: lane l owns
 // i = l, l+64, ..., adds them in that order, then a xor-butterfly combines the
@@ -918,6 +1155,35 @@ static int col_tile(const FftPlan &pl)
     return t;
 }
 
+// The register-resident kernels need radices 8/4/2 only and one thread per
+// 8 elements: n a multiple of 8 with no radix-3 stage.
+static bool plan_regs(const FftPlan &pl)
+{
+    if (pl.n % 8 != 0 || pl.n < 16) return false;
+    for (int s = 0; s < pl.nst; s++)
+        if (pl.R[s] != 8 && pl.R[s] != 4 && pl.R[s] != 2) return false;
+    return true;
+}
+
+// x-line pairs per block (register path): threads = p*n/8 <= 1024, LDS <= LDS_CAP.
+static int rpairs(const FftPlan &pl)
+{
+    int p = 8;
+    while (p > 1 && p * (pl.n / 8) > FFT_MAX_THREADS) p >>= 1;
+    while (p > 1 && (size_t)p * rline_stride(pl.n) * sizeof(double2) > LDS_CAP) p >>= 1;
+    while (p > 1 && (pl.n * pl.n / 2) % p != 0) p >>= 1;
+    return p;
+}
+
+// kx columns per block (register path).
+static int rtile(const FftPlan &pl)
+{
+    int t = 8;
+    while (t > 1 && t * (pl.n / 8) > FFT_MAX_THREADS) t >>= 1;
+    while (t > 1 && (size_t)t * pl.n * sizeof(double2) > LDS_CAP) t >>= 1;
+    return t;
+}
+
 static int fft_threads(const FftPlan &pl, int nlines)
 {
     int t = nlines * (pl.n / plan_rmin(pl));
@@ -927,6 +1193,19 @@ static int fft_threads(const FftPlan &pl, int nlines)
 
 void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
 {
+    if (plan_regs(pl)) {
+        const int ppb = rpairs(pl), pairs = c.n * c.n / 2;
+        const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
+        const dim3 grid(pairs / ppb), blk(ppb * (c.n / 8));
+        if (c.mixed)
+            hipLaunchKernelGGL(k_xfwd_r<float>, grid, blk, shm, s, (const float *)f.us, (const float *)f.vs,
+                               (const float *)f.ws, f.spec, f.tw, f.sc, c, pl, ppb, f.ld);
+        else
+            hipLaunchKernelGGL(k_xfwd_r<double>, grid, blk, shm, s, f.us, f.vs, f.ws, f.spec, f.tw, f.sc, c, pl,
+                               ppb, f.ld);
+        li->name = "xfwd_div"; li->bytes = (c.mixed ? 2.5 : 4.0) * F(c);
+        return;
+    }
     int ppb = pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
@@ -942,6 +1221,18 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
 void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse, hipStream_t s,
                   LaunchInfo *li)
 {
+    if (plan_regs(pl)) {
+        const int tkx = rtile(pl), nh = c.n / 2 + 1;
+        const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
+        const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+        if (inverse)
+            hipLaunchKernelGGL(k_ypass_r<1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        else
+            hipLaunchKernelGGL(k_ypass_r<-1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        li->name = inverse ? "y_inv" : "y_fwd";
+        li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
+        return;
+    }
     int tkx = col_tile(pl);
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
@@ -957,6 +1248,15 @@ void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inver
 
 void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
 {
+    if (plan_regs(pl)) {
+        const int tkx = rtile(pl), nh = c.n / 2 + 1;
+        const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
+        const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+        hipLaunchKernelGGL(k_zpoisson_r, grid, blk, shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+        li->name = "z_poisson";
+        li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
+        return;
+    }
     int tkx = col_tile(pl);
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
@@ -969,6 +1269,14 @@ void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStr
 
 void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
 {
+    if (plan_regs(pl)) {
+        const int ppb = rpairs(pl), pairs = c.n * c.n / 2;
+        const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
+        hipLaunchKernelGGL(k_xinv_r, dim3(pairs / ppb), dim3(ppb * (c.n / 8)), shm, s, f.spec, f.p, f.tw, f.sc, pl,
+                           ppb, f.ld);
+        li->name = "x_inv"; li->bytes = 16.0 * (double)(c.n / 2 + 1) * c.n * c.n + F(c);
+        return;
+    }
     int ppb = pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
