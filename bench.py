@@ -9,6 +9,9 @@ are resident in HBM before the timed region.  One step = every kernel of
 multi/main.f90:329-946 (phase field, momentum + CSF + AB2, divergence, FFT
 Poisson solve, projection, mean removal, umax/CFL).
 
+`value` is the FP64 step (the drop-in); the same inputs are then run in mixed
+precision (MHIT_MIXED) and reported under "mixed".
+
 Prints ONE JSON line.  --gpus above 1 is refused: this build targets one GPU.
 """
 import argparse
@@ -93,6 +96,7 @@ def main():
     ap.add_argument("--phiflag", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -124,25 +128,36 @@ def main():
     print(f"[bench] inputs {n}^3 ready in {time.perf_counter() - t_ic:.1f}s, dt={dt:.4e}",
           file=sys.stderr, flush=True)
 
-    s = Solver(**params_for(n, phiflag, dt))
-    for f, a in fields.items():
-        s.set_field(f, a)
-    s.step(args.warmup)
-    if not args.no_kernel_timing:
-        s.kernel_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    d = s.step(args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats = s.kernel_stats() if not args.no_kernel_timing else []
-    s.kernel_timing(False)
-    dev_bytes = s.device_bytes()
-    s.close()
-    ms = elapsed * 1e3 / args.steps
+    def timed_run(precision, kernel_timing):
+        s = Solver(**params_for(n, phiflag, dt), precision=precision)
+        for f, a in fields.items():
+            s.set_field(f, a)
+        s.step(args.warmup)
+        if kernel_timing:
+            s.kernel_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d = s.step(args.steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        stats = s.kernel_stats() if kernel_timing else []
+        s.kernel_timing(False)
+        dev_bytes = s.device_bytes()
+        s.close()
+        return elapsed * 1e3 / args.steps, d, stats, dev_bytes
+
+    ms, d, stats, dev_bytes = timed_run(0, not args.no_kernel_timing)
     balg = alg_bytes_per_step(n, phiflag)
-    print(f"[bench] {args.steps} steps: {ms:.3f} ms/step, gumax={d.gumax:.4f} cou={d.cou:.3f}",
+    print(f"[bench] FP64 {args.steps} steps: {ms:.3f} ms/step, gumax={d.gumax:.4f} cou={d.cou:.3f}",
           file=sys.stderr, flush=True)
+    mixed = None
+    if not args.no_mixed:
+        ms_m, d_m, stats_m, bytes_m = timed_run(1, not args.no_kernel_timing)
+        print(f"[bench] mixed {args.steps} steps: {ms_m:.3f} ms/step", file=sys.stderr, flush=True)
+        mixed = {"ms_per_step": round(ms_m, 4), "speedup_vs_fp64": round(ms / ms_m, 3),
+                 "device_bytes": bytes_m, "gumax": d_m.gumax,
+                 "kernels": [{"name": k["name"], "avg_ms": round(k["avg_ms"], 4)} for k in stats_m],
+                 "storage": "u,v,w,u*,rhs_o,phi FP32; p, spectrum, reductions FP64"}
 
     roof = None
     kernels = []
@@ -186,6 +201,7 @@ def main():
         "device_bytes": dev_bytes,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

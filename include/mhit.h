@@ -88,6 +88,14 @@ typedef struct mhit_diag {
     double ms;       /* wall time of the call, milliseconds                        */
 } mhit_diag;
 
+/* On-device diagnostics of the current fields. */
+typedef struct mhit_stats {
+    double ke;          /* 0.5 * mean(u^2 + v^2 + w^2)                            */
+    double diss;        /* 2 nu <S_ij S_ij>, post/compute_diss/main.f90:61-117    */
+    double phi_sum;     /* sum of phi over the cells (0 without the phase field)   */
+    double phi_volume;  /* phi_sum * dx^3                                          */
+} mhit_stats;
+
 /* Per-kernel timing collected while kernel timing is on. */
 typedef struct mhit_kstat {
     char name[32];
@@ -137,6 +145,11 @@ int mhit_reset_time(mhit_ctx *ctx, int tstart);
 int mhit_write_field(mhit_ctx *ctx, const char *dir, int t, int fieldn);
 /* readfield / readfield_restart (readwrite.f90:107-318): read <path> into a field. */
 int mhit_read_field(mhit_ctx *ctx, const char *path, int fieldn);
+
+/* Diagnostics (no counterpart in the solver: the reference computes them offline
+ * in post/compute_diss from dumped fields).  dxi <= 0 uses the solver's 1/dx;
+ * compute_diss itself uses dxi = (N-1)/(2*pi) with a REAL(4) pi. */
+int mhit_diagnostics(mhit_ctx *ctx, double dxi, double nu, mhit_stats *out);
 
 /* Kernel timing with HIP events on the context's stream. */
 int mhit_kernel_timing(mhit_ctx *ctx, int enable);

@@ -30,7 +30,7 @@ EXPORTS = (
     "mhit_default_params", "mhit_read_input", "mhit_derive", "mhit_create",
     "mhit_destroy", "mhit_init_fields", "mhit_set_field", "mhit_get_field",
     "mhit_step", "mhit_reset_time", "mhit_write_field", "mhit_read_field",
-    "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
+    "mhit_diagnostics", "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
     "mhit_debug_math", "mhit_debug_buffer", "mhit_strerror", "mhit_abi_version",
 )
 
@@ -66,6 +66,10 @@ class Diag(ctypes.Structure):
         ("umean", ctypes.c_double), ("vmean", ctypes.c_double), ("wmean", ctypes.c_double),
         ("ms", ctypes.c_double),
     ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_double) for name in ("ke", "diss", "phi_sum", "phi_volume")]
 
 
 class KStat(ctypes.Structure):
@@ -109,6 +113,7 @@ def load():
         "mhit_reset_time": ([vp, ctypes.c_int], ctypes.c_int),
         "mhit_write_field": ([vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mhit_read_field": ([vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "mhit_diagnostics": ([vp, ctypes.c_double, ctypes.c_double, P(Stats)], ctypes.c_int),
         "mhit_kernel_timing": ([vp, ctypes.c_int], ctypes.c_int),
         "mhit_kernel_stats": ([vp, P(KStat), ctypes.c_int, P(ctypes.c_int)], ctypes.c_int),
         "mhit_device_bytes": ([vp, P(ctypes.c_size_t)], ctypes.c_int),

@@ -58,6 +58,7 @@ struct Fields {
     double *ktab;                // wavenumber table
     double2 *tw;                 // FFT twiddles
     Scalars *sc;
+    double *diag;                // 3 diagnostic sums (kinetic, strain, phase)
     int ld;                      // spectrum leading dimension (>= n/2+1)
 };
 
@@ -81,5 +82,6 @@ void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo 
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s);
+void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s);
 
 }  // namespace mhit

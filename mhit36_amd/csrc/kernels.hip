@@ -1049,6 +1049,75 @@ __global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum
     }
 }
 
+// ---------------------------------------------------------------------------
+// On-device diagnostics: kinetic energy 0.5<u.u>, the dissipation
+// 2 nu <S_ij S_ij> of post/compute_diss/main.f90:61-117 (central differences
+// on the collocated grid with a caller-given 1/dx) and the phase sum.  One
+// wavefront per x-line, then per plane, then one for the total.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_diag_lines(const T *__restrict__ u, const T *__restrict__ v,
+                                                    const T *__restrict__ w, const T *__restrict__ phi,
+                                                    double *__restrict__ lsum, const Scalars *__restrict__ sc,
+                                                    Consts c, double dxi)
+{
+    const int n = c.n;
+    const int lane = threadIdx.x & 63;
+    const size_t line = ((size_t)blockIdx.x * BLK + threadIdx.x) >> 6;
+    if (line >= (size_t)n * n) return;
+    const int j = (int)(line % n), k = (int)(line / n);
+    const int jp = j + 1 == n ? 0 : j + 1, jm = j == 0 ? n - 1 : j - 1;
+    const int kp = k + 1 == n ? 0 : k + 1, km = k == 0 ? n - 1 : k - 1;
+    const double um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    double ke = 0.0, ss = 0.0, ph = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const int ip = i + 1 == n ? 0 : i + 1, im = i == 0 ? n - 1 : i - 1;
+#define DU(f, m, a, b, cc) ((double)f[IDX(a, b, cc)] - m)
+        const double u0 = DU(u, um, i, j, k), v0 = DU(v, vm, i, j, k), w0 = DU(w, wm, i, j, k);
+        const double dudx = (DU(u, um, ip, j, k) - DU(u, um, im, j, k)) * 0.5 * dxi;
+        const double dudy = (DU(u, um, i, jp, k) - DU(u, um, i, jm, k)) * 0.5 * dxi;
+        const double dudz = (DU(u, um, i, j, kp) - DU(u, um, i, j, km)) * 0.5 * dxi;
+        const double dvdx = (DU(v, vm, ip, j, k) - DU(v, vm, im, j, k)) * 0.5 * dxi;
+        const double dvdy = (DU(v, vm, i, jp, k) - DU(v, vm, i, jm, k)) * 0.5 * dxi;
+        const double dvdz = (DU(v, vm, i, j, kp) - DU(v, vm, i, j, km)) * 0.5 * dxi;
+        const double dwdx = (DU(w, wm, ip, j, k) - DU(w, wm, im, j, k)) * 0.5 * dxi;
+        const double dwdy = (DU(w, wm, i, jp, k) - DU(w, wm, i, jm, k)) * 0.5 * dxi;
+        const double dwdz = (DU(w, wm, i, j, kp) - DU(w, wm, i, j, km)) * 0.5 * dxi;
+#undef DU
+        const double s12 = 0.5 * (dudy + dvdx), s13 = 0.5 * (dudz + dwdx), s23 = 0.5 * (dvdz + dwdy);
+        ss = ss + dudx * dudx + dvdy * dvdy + dwdz * dwdz + 2.0 * ((s12 * s12 + s13 * s13) + s23 * s23);
+        ke = ke + ((u0 * u0 + v0 * v0) + w0 * w0);
+        if (phi) ph = ph + (double)phi[IDX(i, j, k)];
+    }
+    ke = wave_sum(ke); ss = wave_sum(ss); ph = wave_sum(ph);
+    if (lane == 0) {
+        const size_t nn = (size_t)n * n;
+        lsum[line] = ke; lsum[nn + line] = ss; lsum[2 * nn + line] = ph;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_diag_planes(const double *__restrict__ lsum, double *__restrict__ psum, int n)
+{
+    const int k = blockIdx.x, comp = blockIdx.y, lane = threadIdx.x;
+    const double *ls = lsum + comp * (size_t)n * n + (size_t)k * n;
+    double a = 0.0;
+    for (int j = lane; j < n; j += 64) a = a + ls[j];
+    a = wave_sum(a);
+    if (lane == 0) psum[comp * n + k] = a;
+}
+
+__global__ void __launch_bounds__(64) k_diag_total(const double *__restrict__ psum, double *__restrict__ out, int n)
+{
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        double a = 0.0;
+        for (int k = lane; k < n; k += 64) a = a + psum[comp * n + k];
+        a = wave_sum(a);
+        if (lane == 0) out[comp] = a;
+    }
+}
+
 // Diagnostic: the device's elementary functions on an array of arguments.
 __global__ void k_debug_math(int op, size_t count, const double *__restrict__ x, double *__restrict__ y)
 {
@@ -1068,6 +1137,19 @@ __global__ void k_debug_math(int op, size_t count, const double *__restrict__ x,
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
+void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s)
+{
+    const size_t threads = (size_t)c.n * c.n * 64;
+    if (c.mixed)
+        hipLaunchKernelGGL(k_diag_lines<float>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, (const float *)f.u,
+                           (const float *)f.v, (const float *)f.w, (const float *)f.phi, f.lsum, f.sc, c, dxi);
+    else
+        hipLaunchKernelGGL(k_diag_lines<double>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, f.u, f.v, f.w, f.phi,
+                           f.lsum, f.sc, c, dxi);
+    hipLaunchKernelGGL(k_diag_planes, dim3(c.n, 3), dim3(64), 0, s, f.lsum, f.psum, c.n);
+    hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(64), 0, s, f.psum, out, c.n);
+}
+
 void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s)
 {
     if (count == 0) return;

@@ -324,6 +324,7 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     TRY(dalloc(c, (void **)&f.ktab, sizeof(double) * n));
     TRY(dalloc(c, (void **)&f.tw, sizeof(double2) * tw.size()));
     TRY(dalloc(c, (void **)&f.sc, sizeof(Scalars)));
+    TRY(dalloc(c, (void **)&f.diag, 4 * sizeof(double)));
 #undef TRY
     if (hipMemcpy(f.sinf, sinf.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(f.cosf, cosf.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
@@ -631,6 +632,24 @@ extern "C" int mhit_read_field(mhit_ctx *c, const char *path, int fieldn)
     fclose(fp);
     if (got != n3) return MHIT_ERR_IO;
     return mhit_set_field(c, fieldn, a.data());
+}
+
+extern "C" int mhit_diagnostics(mhit_ctx *c, double dxi, double nu, mhit_stats *out)
+{
+    if (!c || !out) return MHIT_ERR_ARG;
+    const int n = c->p.nx;
+    const double dx = dxi > 0.0 ? dxi : c->cs.dxi;
+    launch_diagnostics(c->f, c->kc, dx, c->f.diag, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double sums[3];
+    HIPCHK(hipMemcpy(sums, c->f.diag, sizeof(sums), hipMemcpyDeviceToHost));
+    const double dn = (double)n;
+    out->ke = 0.5 * (sums[0] / dn / dn / dn);
+    out->diss = 2.0 * nu * sums[1] / dn / dn / dn;
+    out->phi_sum = c->kc.phiflag ? sums[2] : 0.0;
+    out->phi_volume = out->phi_sum * c->cs.dx * c->cs.dx * c->cs.dx;
+    return MHIT_OK;
 }
 
 extern "C" int mhit_debug_math(int op, size_t count, const double *x, double *y)

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import Params, Consts, Diag, KStat, check, MhitError
+from ._lib import Params, Consts, Diag, KStat, Stats, check, MhitError
 
 FIELD_NAMES = {1: "u", 2: "v", 3: "w", 4: "p", 5: "phi"}   # readwrite.f90:21-25
 
@@ -135,6 +135,15 @@ class Solver:
 
     def read_field(self, path, fieldn):
         check(self._lib.mhit_read_field(self._h, os.fsencode(path), int(fieldn)), f"read_field({path})")
+
+    # -- diagnostics ------------------------------------------------------------
+    def diagnostics(self, nu=None, dxi=0.0):
+        """Kinetic energy, dissipation 2 nu <S:S> (compute_diss formula) and the
+        phase sum of the current fields, reduced on the device."""
+        st = Stats()
+        nu = self.params.mu / self.params.rho if nu is None else nu
+        check(self._lib.mhit_diagnostics(self._h, float(dxi), float(nu), ctypes.byref(st)), "diagnostics")
+        return {name: getattr(st, name) for name, _ in Stats._fields_}
 
     # -- measurement ------------------------------------------------------------
     def kernel_timing(self, enable=True):
