@@ -166,6 +166,48 @@ int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
 int mhit_debug_math(int op, size_t count, const double *x, double *y);
 int mhit_debug_buffer(mhit_ctx *ctx, int which, double *host);
 
+/* ------------------------------------------------------------------------- */
+/* Ensemble serving (no counterpart in the reference, which runs one box per  */
+/* job): independent boxes of uneven size admitted by an HBM budget, kept     */
+/* resident, stepped concurrently (one stream and replayed hipGraph per box). */
+/* ------------------------------------------------------------------------- */
+typedef struct mhit_ens mhit_ens;
+
+typedef struct mhit_ens_config {
+    size_t hbm_budget_bytes;  /* device bytes the resident boxes may use (0: 80% of free) */
+    int max_active;           /* boxes resident at once (0: 64)                          */
+    int inflight;             /* steps queued ahead per box (0: 2)                       */
+    int use_graphs;           /* 1: replay each box's step as a captured hipGraph        */
+    int reserved;
+} mhit_ens_config;
+
+typedef struct mhit_ens_result {
+    long long id;
+    int status;               /* MHIT_OK, MHIT_ERR_CFL, MHIT_ERR_NAN or a setup error  */
+    int steps_done;
+    int nx;
+    int reserved;
+    double submit_ms, admit_ms, finish_ms;   /* ensemble clock, milliseconds          */
+    double gumax, cou;
+} mhit_ens_result;
+
+int mhit_ens_create(const mhit_ens_config *cfg, mhit_ens **ens);
+int mhit_ens_destroy(mhit_ens *ens);
+/* Queue a box: parameters, optional initial u, v, w (and phi) interior fields
+ * (NULL: the built-in IC of mhit_init_fields), and the number of steps. */
+int mhit_ens_submit(mhit_ens *ens, const mhit_params *p, const double *u, const double *v, const double *w,
+                    const double *phi, int nsteps, long long *id);
+/* One non-blocking scheduling pass: admit, retire finished steps, queue new ones. */
+int mhit_ens_progress(mhit_ens *ens, int *n_active, int *n_queued);
+int mhit_ens_wait_idle(mhit_ens *ens);
+/* Results of finished requests not reported before. */
+int mhit_ens_results(mhit_ens *ens, mhit_ens_result *out, int max_entries, int *n_entries);
+/* Final interior field (fieldn 1..5) of a finished request, kept on the host. */
+int mhit_ens_get_field(mhit_ens *ens, long long id, int fieldn, double *host);
+int mhit_ens_release(mhit_ens *ens, long long id);
+/* Per-step device latency (stream events around each step) of all steps so far. */
+int mhit_ens_latency(mhit_ens *ens, double *p50_ms, double *p99_ms, double *mean_ms, long long *count);
+
 const char *mhit_strerror(int code);
 int mhit_abi_version(void);
 

@@ -31,7 +31,10 @@ EXPORTS = (
     "mhit_destroy", "mhit_init_fields", "mhit_set_field", "mhit_get_field",
     "mhit_step", "mhit_reset_time", "mhit_write_field", "mhit_read_field",
     "mhit_diagnostics", "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
-    "mhit_debug_math", "mhit_debug_buffer", "mhit_strerror", "mhit_abi_version",
+    "mhit_debug_math", "mhit_debug_buffer",
+    "mhit_ens_create", "mhit_ens_destroy", "mhit_ens_submit", "mhit_ens_progress", "mhit_ens_wait_idle",
+    "mhit_ens_results", "mhit_ens_get_field", "mhit_ens_release", "mhit_ens_latency",
+    "mhit_strerror", "mhit_abi_version",
 )
 
 

@@ -31,6 +31,17 @@ struct Scalars {
     int pad;
 };
 
+// Progress record of one ensemble box in host-mapped memory: the step count
+// and a ring of device wall-clock stamps, one per finished step, written by
+// the step's last kernel so the host polls it without any HIP call.
+constexpr int CLK_RING = 4096;
+struct EnsSlot {
+    volatile int steps;
+    volatile int halted;
+    int pad[2];
+    volatile unsigned long long clk[CLK_RING];
+};
+
 // Physical constants needed by the kernels (readinput.f90:35-43 + input.inp).
 struct Consts {
     int n;
@@ -59,6 +70,7 @@ struct Fields {
     double2 *tw;                 // FFT twiddles
     Scalars *sc;
     double *diag;                // 3 diagnostic sums (kinetic, strain, phase)
+    EnsSlot *slot = nullptr;     // host-mapped progress record (ensemble boxes only)
     int ld;                      // spectrum leading dimension (>= n/2+1)
 };
 

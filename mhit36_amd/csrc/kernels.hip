@@ -1012,7 +1012,7 @@ __global__ void __launch_bounds__(64) k_reduce_planes(const double *__restrict__
 
 // Means (main.f90:906-913), signed umax (937-941), CFL (942-945), halt flag.
 __global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum, const double *__restrict__ pmax,
-                                                 Scalars *__restrict__ sc, Consts c)
+                                                 Scalars *__restrict__ sc, Consts c, EnsSlot *slot)
 {
     if (sc->halted) return;
     const int n = c.n, lane = threadIdx.x;
@@ -1043,9 +1043,17 @@ __global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum
         sc->wmean = gw;
         sc->gumax = gumax;
         sc->cou = cou;
-        sc->steps = sc->steps + 1;
-        if (nan) { sc->halted = 1; sc->status = 6; }
-        else if (cou > 7.0) { sc->halted = 1; sc->status = 5; }
+        const int steps = sc->steps + 1;
+        sc->steps = steps;
+        int halt = 0;
+        if (nan) { sc->halted = 1; sc->status = 6; halt = 1; }
+        else if (cou > 7.0) { sc->halted = 1; sc->status = 5; halt = 1; }
+        if (slot) {
+            slot->clk[steps % CLK_RING] = wall_clock64();
+            if (halt) slot->halted = 1;
+            __threadfence_system();
+            slot->steps = steps;
+        }
     }
 }
 
@@ -1388,7 +1396,7 @@ void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *
 
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, f.psum, f.pmax, f.sc, c);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, f.psum, f.pmax, f.sc, c, f.slot);
     li->name = "finalize"; li->bytes = 6.0 * 8.0 * c.n;
 }
 

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,57 @@ struct KStat {
     double bytes = 0.0;
 };
 
+// First-fit sub-allocator over one device allocation (the ensemble's HBM
+// budget), so that admitting and retiring boxes never calls hipMalloc/hipFree
+// (hipFree waits for the whole device).  256-byte aligned blocks.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;      // offset -> length
+    std::map<size_t, size_t> used_;      // offset -> length
+    size_t in_use = 0;
+    std::mutex mu;                       // boxes are built and freed on several threads
+
+    void *alloc(size_t bytes)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            size_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            used_[off] = bytes;
+            in_use += bytes;
+            return base + off;
+        }
+        return nullptr;
+    }
+    void release(void *p)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto u = used_.find((size_t)((char *)p - base));
+        if (u == used_.end()) return;
+        size_t off = u->first, len = u->second;
+        used_.erase(u);
+        in_use -= len;
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + len == nx->first) {
+            len += nx->second;
+            free_.erase(nx);
+        }
+        auto pv = free_.lower_bound(off);
+        if (pv != free_.begin()) {
+            --pv;
+            if (pv->first + pv->second == off) {
+                pv->second += len;
+                return;
+            }
+        }
+        free_[off] = len;
+    }
+};
+
 struct mhit_ctx {
     mhit_params p;
     mhit_consts cs;
@@ -45,6 +97,8 @@ struct mhit_ctx {
     int halt_status = 0;
     std::vector<void *> allocs;
     size_t bytes = 0;
+    Arena *arena = nullptr;      // sub-allocate from here instead of hipMalloc
+    bool own_stream = true;
     std::vector<double> x;       // grid, main.f90:149-153
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
@@ -213,23 +267,30 @@ static int dalloc(mhit_ctx *c, void **ptr, size_t bytes)
 {
     *ptr = nullptr;
     if (bytes == 0) bytes = 256;
-    if (hipMalloc(ptr, bytes) != hipSuccess) {
+    if (c->arena) {
+        *ptr = c->arena->alloc(bytes);
+        if (!*ptr) return MHIT_ERR_OOM;
+    } else if (hipMalloc(ptr, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return MHIT_ERR_OOM;
     }
     c->allocs.push_back(*ptr);
     c->bytes += bytes;
-    HIPCHK(hipMemset(*ptr, 0, bytes));
+    HIPCHK(hipMemsetAsync(*ptr, 0, bytes, c->stream));
     return MHIT_OK;
 }
 
 static void free_all(mhit_ctx *c)
 {
-    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) {
+        if (c->arena) c->arena->release(p);
+        else (void)hipFree(p);
+    }
     c->allocs.clear();
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
 
@@ -242,7 +303,15 @@ extern "C" int mhit_destroy(mhit_ctx *ctx)
     return MHIT_OK;
 }
 
+static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipStream_t stream);
+
 extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
+{
+    return create_impl(p, out, nullptr, nullptr);
+}
+
+// A context on `stream` (own stream when null) with memory from `arena` (hipMalloc when null).
+static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipStream_t stream)
 {
     if (!p || !out) return MHIT_ERR_ARG;
     *out = nullptr;
@@ -270,7 +339,11 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
         }                           \
     } while (0)
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MHIT_ERR_HIP; }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->arena = arena;
+    if (stream) {
+        c->stream = stream;
+        c->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return MHIT_ERR_HIP;
     }
@@ -326,17 +399,16 @@ extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
     TRY(dalloc(c, (void **)&f.sc, sizeof(Scalars)));
     TRY(dalloc(c, (void **)&f.diag, 4 * sizeof(double)));
 #undef TRY
-    if (hipMemcpy(f.sinf, sinf.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(f.cosf, cosf.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(f.ktab, ktab.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(f.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess) {
-        mhit_destroy(c);
-        return MHIT_ERR_HIP;
-    }
     Scalars s0;
     memset(&s0, 0, sizeof(s0));
     s0.gumax = 1.0;                 // main.f90:325
-    if (hipMemcpy(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) {
+    hipStream_t st = c->stream;
+    if (hipMemcpyAsync(f.sinf, sinf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(f.cosf, cosf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(f.ktab, ktab.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(f.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
         mhit_destroy(c);
         return MHIT_ERR_HIP;
     }
@@ -356,10 +428,12 @@ static double *field_ptr(mhit_ctx *c, int fieldn)
     }
 }
 
+// Copies go through the context's own stream so that other contexts (other
+// boxes of an ensemble) keep running.
 static int read_scalars(mhit_ctx *c, Scalars *s)
 {
+    HIPCHK(hipMemcpyAsync(s, c->f.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(s, c->f.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     return MHIT_OK;
 }
 
@@ -369,20 +443,21 @@ extern "C" int mhit_set_field(mhit_ctx *c, int fieldn, const double *host)
     double *d = field_ptr(c, fieldn);
     if (!d) return MHIT_ERR_ARG;
     const size_t n = c->p.nx, n3 = n * n * n;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<float> tmp;
     if (c->kc.mixed && fieldn != 4) {
-        std::vector<float> tmp(n3);
+        tmp.resize(n3);
         for (size_t q = 0; q < n3; q++) tmp[q] = (float)host[q];
-        HIPCHK(hipMemcpy(d, tmp.data(), n3 * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(d, tmp.data(), n3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
     } else {
-        HIPCHK(hipMemcpy(d, host, n3 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(d, host, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
+    double zero = 0.0;
     if (fieldn <= 3) {
         // the stored field is now the field itself: its pending mean is zero
-        double zero = 0.0;
         double *m = &c->f.sc->umean + (fieldn - 1);
-        HIPCHK(hipMemcpy(m, &zero, sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(m, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
+    HIPCHK(hipStreamSynchronize(c->stream));
     return MHIT_OK;
 }
 
@@ -397,10 +472,12 @@ extern "C" int mhit_get_field(mhit_ctx *c, int fieldn, double *host)
     if (rc) return rc;
     if (c->kc.mixed && fieldn != 4) {
         std::vector<float> tmp(n3);
-        HIPCHK(hipMemcpy(tmp.data(), d, n3 * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(tmp.data(), d, n3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         for (size_t q = 0; q < n3; q++) host[q] = (double)tmp[q];
     } else {
-        HIPCHK(hipMemcpy(host, d, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(host, d, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     if (fieldn <= 3) {
         // u - gumean (main.f90:917-919), the rounding every kernel applies
@@ -421,14 +498,20 @@ extern "C" int mhit_init_fields(mhit_ctx *c)
     std::vector<double> a(n3);
     int rc;
     if (c->p.inflow == 0) {
+        // sin/cos of each grid coordinate once; the products keep the
+        // left-to-right order of main.f90:227-228, so the values are the same
+        std::vector<double> sn(n), cs(n);
+        for (int i = 0; i < n; i++) {
+            sn[i] = sin(x[i]);
+            cs[i] = cos(x[i]);
+        }
         for (int k = 0; k < n; k++)
             for (int j = 0; j < n; j++)
-                for (int i = 0; i < n; i++) a[i + (size_t)n * (j + (size_t)n * k)] = sin(x[i]) * cos(x[j]) * cos(x[k]);
+                for (int i = 0; i < n; i++) a[i + (size_t)n * (j + (size_t)n * k)] = sn[i] * cs[j] * cs[k];
         if ((rc = mhit_set_field(c, 1, a.data()))) return rc;
         for (int k = 0; k < n; k++)
             for (int j = 0; j < n; j++)
-                for (int i = 0; i < n; i++)
-                    a[i + (size_t)n * (j + (size_t)n * k)] = -(cos(x[i]) * sin(x[j]) * cos(x[k]));
+                for (int i = 0; i < n; i++) a[i + (size_t)n * (j + (size_t)n * k)] = -(cs[i] * sn[j] * cs[k]);
         if ((rc = mhit_set_field(c, 2, a.data()))) return rc;
         std::fill(a.begin(), a.end(), 0.0);
         if ((rc = mhit_set_field(c, 3, a.data()))) return rc;
@@ -641,9 +724,9 @@ extern "C" int mhit_diagnostics(mhit_ctx *c, double dxi, double nu, mhit_stats *
     const double dx = dxi > 0.0 ? dxi : c->cs.dxi;
     launch_diagnostics(c->f, c->kc, dx, c->f.diag, c->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
     double sums[3];
-    HIPCHK(hipMemcpy(sums, c->f.diag, sizeof(sums), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(sums, c->f.diag, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     const double dn = (double)n;
     out->ke = 0.5 * (sums[0] / dn / dn / dn);
     out->diss = 2.0 * nu * sums[1] / dn / dn / dn;
@@ -727,3 +810,514 @@ extern "C" const char *mhit_strerror(int code)
 }
 
 extern "C" int mhit_abi_version(void) { return MHIT_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// Ensemble serving: many independent boxes resident at once
+//
+// Requests wait in a FIFO and are admitted while the device bytes of the
+// resident boxes stay within the HBM budget and below max_active.  Every
+// resident box owns a context (its own stream); its step is replayed as a
+// hipGraph captured once per step variant (Euler first step, AB2 with each
+// phi buffer parity), and up to `inflight` steps per box are queued ahead, so
+// the boxes' kernels overlap on the device.
+//
+// The caller's thread (mhit_ens_progress) only decides admissions, reads each
+// box's step counter from host-mapped memory and queues steps, so it never
+// blocks on a copy.  A few I/O threads owned by the ensemble do everything
+// that waits: building a context, uploading its fields and capturing its
+// graphs (admission), and synchronising, copying the final fields to host
+// memory and freeing the device memory (retirement).  Retirements go ahead of
+// admissions, since they give memory back.
+// ---------------------------------------------------------------------------
+#include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+
+namespace {
+
+struct EnsReq {
+    long long id = 0;
+    mhit_params p{};
+    std::vector<double> in[4];           // u, v, w, phi (empty: built-in IC)
+    int nsteps = 0;
+    double submit_ms = 0, admit_ms = 0, finish_ms = 0;
+    mhit_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;        // assigned by the caller's thread
+    double *phi0 = nullptr, *phin0 = nullptr;   // phase buffers at admission
+    size_t bytes = 0;
+    hipGraphExec_t g_euler = nullptr, g_ab2[2] = {nullptr, nullptr};
+    int launched = 0, done = 0;
+    EnsSlot *slot = nullptr;             // host-mapped progress record
+    int status = MHIT_OK;
+    int job_rc = MHIT_OK;                // result of the last I/O-thread job
+    double gumax = 0, cou = 0;
+    std::vector<double> out[5];          // final u, v, w, p, phi
+    bool finished = false, reported = false;
+};
+
+enum JobKind { JOB_ADMIT, JOB_RETIRE };
+
+constexpr int ENS_IO_THREADS = 4;
+
+struct Job {
+    JobKind kind;
+    EnsReq *r;
+};
+
+}  // namespace
+
+struct mhit_ens {
+    mhit_ens_config cfg{};
+    Arena arena;
+    int device = 0;
+    std::vector<hipStream_t> streams;    // idle streams for new boxes
+    std::deque<EnsReq *> queue;          // submitted, not yet admitted
+    std::vector<EnsReq *> active;        // stepping
+    int admitting = 0, retiring = 0;     // boxes held by the I/O thread
+    std::map<long long, std::unique_ptr<EnsReq>> all;
+    long long next_id = 1;
+    size_t resident = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<double> lat_ms;
+    std::vector<EnsSlot *> free_slots;
+    std::vector<EnsSlot *> all_slots;
+    double clk_khz = 0.0;                // device wall clock rate
+    // I/O threads
+    std::vector<std::thread> io;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job> jobs;                // to the I/O thread
+    std::vector<Job> done_jobs;          // back from it
+    bool stop = false;
+};
+
+static double ens_now(mhit_ens *e)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - e->t0).count();
+}
+
+// Device bytes mhit_create allocates for these parameters.
+static size_t ctx_bytes(const mhit_params *p)
+{
+    const size_t n = p->nx, n3 = n * n * n, F = n3 * 8;
+    const size_t FT = n3 * (p->precision == MHIT_MIXED ? 4 : 8);
+    const size_t ld = ((n / 2 + 1) + 7) / 8 * 8;
+    size_t b = 9 * FT + F + (p->phiflag ? 2 * FT : 0) + 16 * n * n * ld;
+    b += 8 * (6 * n * n + 6 * n + 4 * n) + 65536;
+    return b;
+}
+
+// Capture one step of `c` in its current state (Euler/AB2, phi parity).
+static int capture_step(mhit_ctx *c, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    const bool timing = c->timing;
+    c->timing = false;
+    double *phi = c->f.phi, *phin = c->f.phin;
+    const int since = c->nsteps_since_init;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    enqueue_step(c);
+    hipError_t e1 = hipStreamEndCapture(c->stream, &g);
+    c->f.phi = phi;
+    c->f.phin = phin;
+    c->nsteps_since_init = since;
+    c->timing = timing;
+    if (e1 != hipSuccess) return MHIT_ERR_HIP;
+    hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    return e2 == hipSuccess ? MHIT_OK : MHIT_ERR_HIP;
+}
+
+static void destroy_graphs(EnsReq *r)
+{
+    hipGraphExec_t *gs[3] = {&r->g_euler, &r->g_ab2[0], &r->g_ab2[1]};
+    for (hipGraphExec_t *g : gs)
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
+// I/O thread: build the box's context on r->stream, upload its fields,
+// capture its step graphs.  On failure the context is already torn down.
+static int io_admit(mhit_ens *e, EnsReq *r)
+{
+    int rc = create_impl(&r->p, &r->ctx, &e->arena, r->stream);
+    if (rc) return rc;
+    mhit_ctx *c = r->ctx;
+    if (r->in[0].empty() || (r->p.phiflag && r->in[3].empty())) rc = mhit_init_fields(c);
+    const int fids[4] = {1, 2, 3, 5};
+    for (int q = 0; q < 4 && rc == MHIT_OK; q++)
+        if (!r->in[q].empty() && (q < 3 || r->p.phiflag)) rc = mhit_set_field(c, fids[q], r->in[q].data());
+    r->phi0 = c->f.phi;
+    r->phin0 = c->f.phin;
+    EnsSlot *dslot = nullptr;
+    if (rc == MHIT_OK && hipHostGetDevicePointer((void **)&dslot, r->slot, 0) != hipSuccess) rc = MHIT_ERR_HIP;
+    c->f.slot = dslot;
+    if (rc == MHIT_OK && e->cfg.use_graphs) {
+        // Euler first step from parity 0, then AB2 steps at parity 1, 0, 1, ...
+        rc = capture_step(c, &r->g_euler);
+        c->nsteps_since_init = 1;
+        if (rc == MHIT_OK && c->kc.phiflag) {
+            std::swap(c->f.phi, c->f.phin);
+            rc = capture_step(c, &r->g_ab2[1]);
+            std::swap(c->f.phi, c->f.phin);
+        }
+        if (rc == MHIT_OK) rc = capture_step(c, &r->g_ab2[0]);
+        c->nsteps_since_init = 0;
+    }
+    if (rc == MHIT_OK) {
+        for (auto &v : r->in) std::vector<double>().swap(v);
+        return MHIT_OK;
+    }
+    destroy_graphs(r);
+    mhit_destroy(c);
+    r->ctx = nullptr;
+    return rc;
+}
+
+// I/O thread: wait for the box's last step, copy its fields out, free it.
+static void io_retire(mhit_ens *e, EnsReq *r)
+{
+    (void)e;
+    mhit_ctx *c = r->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    Scalars s;
+    if (read_scalars(c, &s) == MHIT_OK) {
+        r->status = s.halted ? s.status : MHIT_OK;
+        r->gumax = s.gumax;
+        r->cou = s.cou;
+        r->done = s.steps;
+    } else {
+        r->status = MHIT_ERR_HIP;
+    }
+    // phi/phin parity of the steps that actually ran (a halted step writes nothing)
+    if (c->kc.phiflag) {
+        c->f.phi = (r->done % 2) ? r->phin0 : r->phi0;
+        c->f.phin = (r->done % 2) ? r->phi0 : r->phin0;
+    }
+    const size_t n3 = (size_t)c->p.nx * c->p.nx * c->p.nx;
+    for (int f = 1; f <= 5; f++) {
+        if (f == 5 && !c->kc.phiflag) continue;
+        r->out[f - 1].resize(n3);
+        if (mhit_get_field(c, f, r->out[f - 1].data()) != MHIT_OK) r->status = MHIT_ERR_HIP;
+    }
+    destroy_graphs(r);
+    mhit_destroy(c);
+    r->ctx = nullptr;
+}
+
+static void io_main(mhit_ens *e)
+{
+    (void)hipSetDevice(e->device);
+    std::unique_lock<std::mutex> lk(e->mu);
+    for (;;) {
+        e->cv.wait(lk, [e] { return e->stop || !e->jobs.empty(); });
+        if (e->jobs.empty()) return;     // stop requested and nothing left
+        Job j = e->jobs.front();
+        e->jobs.pop_front();
+        lk.unlock();
+        if (j.kind == JOB_ADMIT) j.r->job_rc = io_admit(e, j.r);
+        else io_retire(e, j.r);
+        lk.lock();
+        e->done_jobs.push_back(j);
+    }
+}
+
+static void post_job(mhit_ens *e, JobKind kind, EnsReq *r)
+{
+    {
+        std::lock_guard<std::mutex> g(e->mu);
+        if (kind == JOB_RETIRE) e->jobs.push_front({kind, r});
+        else e->jobs.push_back({kind, r});
+    }
+    e->cv.notify_one();
+}
+
+// Step service times of a finished box: gaps between consecutive step
+// completions on the device clock (the host keeps `inflight` steps queued,
+// so the next step is already waiting when one ends).
+static void record_latency(mhit_ens *e, EnsReq *r)
+{
+    const int last = r->slot->steps;
+    const int first = std::max(2, last - CLK_RING + 2);
+    for (int q = first; q <= last; q++) {
+        unsigned long long a = r->slot->clk[(q - 1) % CLK_RING], b = r->slot->clk[q % CLK_RING];
+        if (b > a && e->clk_khz > 0) e->lat_ms.push_back((double)(b - a) / e->clk_khz);
+    }
+}
+
+extern "C" int mhit_ens_create(const mhit_ens_config *cfg, mhit_ens **out)
+{
+    if (!out) return MHIT_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return MHIT_ERR_NODEV;
+    }
+    mhit_ens *e = new mhit_ens();
+    if (cfg) e->cfg = *cfg;
+    if (e->cfg.max_active <= 0) e->cfg.max_active = 64;
+    if (e->cfg.inflight <= 0) e->cfg.inflight = 2;
+    if (e->cfg.hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return MHIT_ERR_HIP; }
+        e->cfg.hbm_budget_bytes = (size_t)(0.8 * (double)fr);
+    }
+    e->cfg.hbm_budget_bytes = (e->cfg.hbm_budget_bytes + 255) & ~(size_t)255;
+    if (hipMalloc((void **)&e->arena.base, e->cfg.hbm_budget_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete e;
+        return MHIT_ERR_OOM;
+    }
+    e->arena.size = e->cfg.hbm_budget_bytes;
+    e->arena.free_[0] = e->arena.size;
+    int khz = 0;
+    if (hipGetDevice(&e->device) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device) == hipSuccess)
+        e->clk_khz = (double)khz;
+    e->t0 = std::chrono::steady_clock::now();
+    for (int q = 0; q < ENS_IO_THREADS; q++) e->io.emplace_back(io_main, e);
+    *out = e;
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_submit(mhit_ens *e, const mhit_params *p, const double *u, const double *v,
+                               const double *w, const double *phi, int nsteps, long long *id)
+{
+    if (!e || !p || nsteps < 0) return MHIT_ERR_ARG;
+    int rad[MAXSTAGES + 4];
+    if (p->nx < 4 || p->nx % 2 != 0 || plan_radices(p->nx, rad) == 0 || p->nx > 2048) return MHIT_ERR_BAD_N;
+    if (p->precision != MHIT_FP64 && p->precision != MHIT_MIXED) return MHIT_ERR_UNSUPPORTED;
+    if (ctx_bytes(p) > e->cfg.hbm_budget_bytes) return MHIT_ERR_OOM;
+    auto r = std::make_unique<EnsReq>();
+    r->id = e->next_id++;
+    r->p = *p;
+    r->nsteps = nsteps;
+    r->bytes = ctx_bytes(p);
+    const size_t n3 = (size_t)p->nx * p->nx * p->nx;
+    const double *src[4] = {u, v, w, phi};
+    if (u && v && w)
+        for (int q = 0; q < 3; q++) r->in[q].assign(src[q], src[q] + n3);
+    if (phi) r->in[3].assign(phi, phi + n3);
+    r->submit_ms = ens_now(e);
+    if (id) *id = r->id;
+    e->queue.push_back(r.get());
+    e->all[r->id] = std::move(r);
+    return MHIT_OK;
+}
+
+// Return a box's stream and progress record to the pools.
+static void give_back(mhit_ens *e, EnsReq *r)
+{
+    e->streams.push_back(r->stream);
+    r->stream = nullptr;
+    e->free_slots.push_back(r->slot);
+    r->slot = nullptr;
+    e->resident -= r->bytes;
+}
+
+extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
+{
+    if (!e) return MHIT_ERR_ARG;
+    // 1. what the I/O thread finished
+    std::vector<Job> done;
+    {
+        std::lock_guard<std::mutex> g(e->mu);
+        done.swap(e->done_jobs);
+    }
+    for (const Job &j : done) {
+        EnsReq *r = j.r;
+        if (j.kind == JOB_RETIRE) {
+            e->retiring--;
+            give_back(e, r);
+            r->finish_ms = ens_now(e);
+            r->finished = true;
+            continue;
+        }
+        e->admitting--;
+        if (r->job_rc == MHIT_OK) {
+            r->admit_ms = ens_now(e);
+            e->active.push_back(r);
+            continue;
+        }
+        give_back(e, r);
+        // a fragmented arena: retry once something else has given memory back
+        if (r->job_rc == MHIT_ERR_OOM && (!e->active.empty() || e->admitting || e->retiring)) {
+            e->queue.push_front(r);
+            continue;
+        }
+        r->status = r->job_rc;
+        r->finished = true;
+        r->finish_ms = ens_now(e);
+    }
+    // 2. admit in arrival order while the budget allows
+    while (!e->queue.empty() && (int)e->active.size() + e->admitting < e->cfg.max_active &&
+           e->resident + e->queue.front()->bytes <= e->cfg.hbm_budget_bytes) {
+        EnsReq *r = e->queue.front();
+        if (e->streams.empty()) {
+            hipStream_t st = nullptr;
+            if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return MHIT_ERR_HIP;
+            e->streams.push_back(st);
+        }
+        if (e->free_slots.empty()) {
+            EnsSlot *sl = nullptr;
+            if (hipHostMalloc((void **)&sl, sizeof(EnsSlot), hipHostMallocMapped | hipHostMallocCoherent) !=
+                hipSuccess)
+                return MHIT_ERR_OOM;
+            e->all_slots.push_back(sl);
+            e->free_slots.push_back(sl);
+        }
+        e->queue.pop_front();
+        r->stream = e->streams.back();
+        e->streams.pop_back();
+        r->slot = e->free_slots.back();
+        e->free_slots.pop_back();
+        r->slot->steps = 0;
+        r->slot->halted = 0;
+        e->resident += r->bytes;
+        e->admitting++;
+        post_job(e, JOB_ADMIT, r);
+    }
+    // 3. read each box's progress record, queue new steps, retire finished boxes
+    for (size_t a = 0; a < e->active.size();) {
+        EnsReq *r = e->active[a];
+        mhit_ctx *c = r->ctx;
+        r->done = r->slot->steps;
+        const bool halted = r->slot->halted != 0;
+        while (!halted && r->launched < r->nsteps && r->launched - r->done < e->cfg.inflight) {
+            if (e->cfg.use_graphs) {
+                hipGraphExec_t g = r->launched == 0 ? r->g_euler
+                                                    : r->g_ab2[c->kc.phiflag ? r->launched % 2 : 0];
+                if (hipGraphLaunch(g, c->stream) != hipSuccess) return MHIT_ERR_HIP;
+            } else {
+                enqueue_step(c);
+            }
+            r->launched++;
+        }
+        if (r->done >= r->nsteps || halted) {
+            record_latency(e, r);
+            e->active.erase(e->active.begin() + a);
+            e->retiring++;
+            post_job(e, JOB_RETIRE, r);
+            continue;
+        }
+        a++;
+    }
+    if (n_active) *n_active = (int)e->active.size() + e->admitting + e->retiring;
+    if (n_queued) *n_queued = (int)e->queue.size();
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_wait_idle(mhit_ens *e)
+{
+    if (!e) return MHIT_ERR_ARG;
+    // Poll with a short sleep between passes: a pure spin holds a core that
+    // the HIP runtime's own threads need when the process has few of them.
+    int act = 1, q = 1;
+    while (act || q) {
+        int rc = mhit_ens_progress(e, &act, &q);
+        if (rc) return rc;
+        if (act || q) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_results(mhit_ens *e, mhit_ens_result *out, int max_entries, int *n_entries)
+{
+    if (!e || !n_entries) return MHIT_ERR_ARG;
+    int k = 0;
+    for (auto &kv : e->all) {
+        EnsReq *r = kv.second.get();
+        if (!r->finished || r->reported) continue;
+        if (!out || k >= max_entries) break;
+        mhit_ens_result &o = out[k++];
+        memset(&o, 0, sizeof(o));
+        o.id = r->id;
+        o.status = r->status;
+        o.steps_done = r->done;
+        o.nx = r->p.nx;
+        o.submit_ms = r->submit_ms;
+        o.admit_ms = r->admit_ms;
+        o.finish_ms = r->finish_ms;
+        o.gumax = r->gumax;
+        o.cou = r->cou;
+        r->reported = true;
+    }
+    *n_entries = k;
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_get_field(mhit_ens *e, long long id, int fieldn, double *host)
+{
+    if (!e || !host || fieldn < 1 || fieldn > 5) return MHIT_ERR_ARG;
+    auto it = e->all.find(id);
+    if (it == e->all.end() || !it->second->finished) return MHIT_ERR_ARG;
+    const std::vector<double> &a = it->second->out[fieldn - 1];
+    if (a.empty()) return MHIT_ERR_ARG;
+    memcpy(host, a.data(), a.size() * sizeof(double));
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_release(mhit_ens *e, long long id)
+{
+    if (!e) return MHIT_ERR_ARG;
+    auto it = e->all.find(id);
+    if (it == e->all.end() || !it->second->finished) return MHIT_ERR_ARG;
+    e->all.erase(it);
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_latency(mhit_ens *e, double *p50, double *p99, double *mean, long long *count)
+{
+    if (!e) return MHIT_ERR_ARG;
+    std::vector<double> v = e->lat_ms;
+    if (count) *count = (long long)v.size();
+    if (v.empty()) {
+        if (p50) *p50 = 0;
+        if (p99) *p99 = 0;
+        if (mean) *mean = 0;
+        return MHIT_OK;
+    }
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) { return v[std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5))]; };
+    double s = 0;
+    for (double x : v) s += x;
+    if (p50) *p50 = pct(0.50);
+    if (p99) *p99 = pct(0.99);
+    if (mean) *mean = s / (double)v.size();
+    return MHIT_OK;
+}
+
+extern "C" int mhit_ens_destroy(mhit_ens *e)
+{
+    if (!e) return MHIT_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> g(e->mu);
+        e->stop = true;
+    }
+    e->cv.notify_all();
+    for (std::thread &t : e->io) t.join();   // they run the jobs already posted first
+    for (EnsReq *r : e->active) {
+        if (!r->ctx) continue;
+        (void)hipStreamSynchronize(r->ctx->stream);
+        destroy_graphs(r);
+        mhit_destroy(r->ctx);
+        r->ctx = nullptr;
+    }
+    for (const Job &j : e->done_jobs)
+        if (j.kind == JOB_ADMIT && j.r->ctx) {
+            destroy_graphs(j.r);
+            mhit_destroy(j.r->ctx);
+            j.r->ctx = nullptr;
+        }
+    for (auto &kv : e->all)
+        if (kv.second->stream) e->streams.push_back(kv.second->stream);
+    for (EnsSlot *sl : e->all_slots) (void)hipHostFree(sl);
+    for (hipStream_t st : e->streams) (void)hipStreamDestroy(st);
+    if (e->arena.base) (void)hipFree(e->arena.base);
+    delete e;
+    return MHIT_OK;
+}

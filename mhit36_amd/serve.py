@@ -1,0 +1,83 @@
+"""Ensemble-serving workload: a seeded Poisson-arrival stream of independent
+periodic boxes (the serving configuration), and the drivers that run it one
+box at a time or through Ensemble (continuous admission, concurrent steps).
+"""
+import time
+
+import numpy as np
+
+from . import ic
+from .ensemble import Ensemble
+from .solver import Solver, default_params
+
+SIZES = (64, 96, 128, 192, 256)
+
+
+def c5_stream(count=32, seed=2024, rate_hz=40.0, steps=(200, 2000), sizes=SIZES, precision=1):
+    """Requests: arrival time (s), parameters, step count and initial phi.
+    N from `sizes`, nu in [0.002, 0.02], half of the boxes with ACDI drops
+    (8-64 drops, radius 0.15-0.3); velocity is the built-in Taylor-Green field,
+    dt = 0.05 dx (CFL 0.05 at umax 1)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for q in range(count):
+        t += rng.exponential(1.0 / rate_hz)
+        n = int(rng.choice(sizes))
+        nu = float(rng.uniform(0.002, 0.02))
+        acdi = bool(rng.random() < 0.5)
+        nsteps = int(np.exp(rng.uniform(np.log(steps[0]), np.log(steps[1]))))
+        _, dx = ic.grid(n)
+        p = dict(nx=n, dt=0.05 * dx, mu=nu, rho=1.0, phiflag=1 if acdi else 0, inflow=0,
+                 inphi=1 if acdi else 0, sigma=0.05, gamma_factor=1.2, precision=precision)
+        phi = None
+        if acdi:
+            nd = int(rng.integers(8, 65))
+            R = float(rng.uniform(0.15, 0.3))
+            p["radius"] = R
+            centres = ic.drop_centres(nd, R, dx, seed=int(rng.integers(1 << 30)))
+            phi = ic.drops(n, centres, R, dx)
+        reqs.append({"arrival": t, "params": p, "nsteps": nsteps, "phi": phi})
+    return reqs
+
+
+def cell_steps(reqs):
+    return float(sum(r["params"]["nx"] ** 3 * r["nsteps"] for r in reqs))
+
+
+def run_one_at_a_time(reqs):
+    """Each box alone: create, run all its steps, copy out, destroy."""
+    t0 = time.perf_counter()
+    out = {}
+    for q, r in enumerate(reqs):
+        with Solver(**r["params"]) as s:
+            s.init_fields()
+            if r["phi"] is not None:
+                s.set_field(5, r["phi"])
+            s.step(r["nsteps"], raise_on_stop=False)
+            out[q] = s.get_field(1)
+    return time.perf_counter() - t0, out
+
+
+def run_ensemble(reqs, follow_arrivals=True, **cfg):
+    """Submit at the arrival times (host clock) and drive the ensemble."""
+    ens = Ensemble(**cfg)
+    t0 = time.perf_counter()
+    ids = {}
+    pending = list(range(len(reqs)))
+    while True:
+        now = time.perf_counter() - t0
+        while pending and (not follow_arrivals or reqs[pending[0]]["arrival"] <= now):
+            q = pending.pop(0)
+            r = reqs[q]
+            ids[q] = ens.submit(default_params(**r["params"]), nsteps=r["nsteps"], phi=r["phi"])
+        active, queued = ens.progress()
+        if not pending and active == 0 and queued == 0:
+            break
+        time.sleep(2e-5)            # leave the core to the HIP runtime's threads
+    wall = time.perf_counter() - t0
+    res = {r["id"]: r for r in ens.results()}
+    out = {q: ens.get_field(i, 1) for q, i in ids.items()}
+    lat = ens.latency()
+    ens.close()
+    return wall, out, lat, [res[ids[q]] for q in range(len(reqs))]

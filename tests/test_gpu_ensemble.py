@@ -1,0 +1,86 @@
+"""Ensemble serving: each box's result in the mix equals the result alone
+(bit for bit: the kernels are deterministic and boxes share no state), HBM
+budget admission, and a stop in one box leaves the others untouched."""
+import numpy as np
+import pytest
+
+from mhit36_amd import Ensemble, Solver, default_params, MHIT_ERR_CFL
+from mhit36_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+BOXES = [
+    dict(nx=16, dt=1e-3, mu=0.01, phiflag=0, precision=0),
+    dict(nx=32, dt=5e-4, mu=0.01, phiflag=1, radius=1.0, sigma=0.1, precision=0),
+    dict(nx=24, dt=1e-3, mu=0.02, phiflag=0, f1=0.5, precision=1),
+    dict(nx=48, dt=5e-4, mu=0.01, phiflag=1, radius=0.8, sigma=0.1, gamma_factor=1.2, precision=1),
+    dict(nx=64, dt=5e-4, mu=0.005, phiflag=1, radius=0.9, sigma=0.05, precision=0),
+]
+STEPS = [7, 12, 5, 9, 6]
+
+
+def alone(p, nsteps):
+    with Solver(**p) as s:
+        s.init_fields()
+        s.step(nsteps)
+        return {f: s.get_field(f) for f in ((1, 2, 3, 4, 5) if p["phiflag"] else (1, 2, 3, 4))}
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_same_alone_and_in_mix(graphs):
+    ref = [alone(p, n) for p, n in zip(BOXES, STEPS)]
+    with Ensemble(use_graphs=graphs, inflight=3) as ens:
+        ids = [ens.submit(default_params(**p), nsteps=n) for p, n in zip(BOXES, STEPS)]
+        ens.wait_idle()
+        res = {r["id"]: r for r in ens.results()}
+        for q, rid in enumerate(ids):
+            assert res[rid]["status"] == 0 and res[rid]["steps_done"] == STEPS[q]
+            for f, a in ref[q].items():
+                assert np.array_equal(ens.get_field(rid, f), a), (q, f)
+        lat = ens.latency()
+        # one latency per step after each box's first (gaps between completions)
+        assert lat["steps"] == sum(STEPS) - len(STEPS) and lat["p99_ms"] >= lat["p50_ms"] > 0
+
+
+def test_budget_admission_serialises():
+    """A budget that holds one 64^3 box at a time admits the next box only
+    after the previous one finished; results are unchanged."""
+    p = dict(nx=64, dt=5e-4, mu=0.005, phiflag=0, precision=0)
+    ref = alone(p, 4)
+    with Ensemble(hbm_budget_bytes=30 << 20, max_active=8) as ens:      # one 64^3 FP64 box is ~23 MB
+        ids = [ens.submit(default_params(**p), nsteps=4) for _ in range(3)]
+        ens.wait_idle()
+        res = sorted(ens.results(), key=lambda r: r["id"])
+        for a, b in zip(res, res[1:]):
+            assert b["admit_ms"] >= a["finish_ms"]
+        for rid in ids:
+            assert np.array_equal(ens.get_field(rid, 1), ref[1])
+
+
+def test_halted_box_does_not_disturb_others():
+    good = dict(nx=32, dt=1e-3, mu=0.01, phiflag=0, precision=0)
+    bad = dict(nx=16, dt=3.0, mu=0.0, phiflag=0, precision=0)
+    ref = alone(good, 10)
+    with Ensemble() as ens:
+        g = ens.submit(default_params(**good), nsteps=10)
+        b = ens.submit(default_params(**bad), nsteps=10)
+        ens.wait_idle()
+        res = {r["id"]: r for r in ens.results()}
+        assert res[b]["status"] == MHIT_ERR_CFL and res[b]["steps_done"] == 1
+        assert res[g]["status"] == 0
+        assert np.array_equal(ens.get_field(g, 1), ref[1])
+
+
+def test_submitted_fields_are_used():
+    n = 32
+    u, v, w = ic.random_velocity(n, seed=3)
+    p = dict(nx=n, dt=1e-3, mu=0.01, phiflag=0, inflow=1, precision=0)
+    with Solver(**p) as s:
+        for f, a in ((1, u), (2, v), (3, w)):
+            s.set_field(f, a)
+        s.step(5)
+        ref = s.get_field(1)
+    with Ensemble() as ens:
+        rid = ens.submit(default_params(**p), nsteps=5, u=u, v=v, w=w)
+        ens.wait_idle()
+        assert np.array_equal(ens.get_field(rid, 1), ref)
