@@ -10,7 +10,9 @@ multi/main.f90:329-946 (phase field, momentum + CSF + AB2, divergence, FFT
 Poisson solve, projection, mean removal, umax/CFL).
 
 `value` is the FP64 step (the drop-in); the same inputs are then run in mixed
-precision (MHIT_MIXED) and reported under "mixed".
+precision (MHIT_MIXED) and reported under "mixed".  "ensemble" is the serving
+stream of independent boxes (ensemble_leg), one at a time against the
+Ensemble loop.
 
 Prints ONE JSON line.  --gpus above 1 is refused: this build targets one GPU.
 """
@@ -87,6 +89,49 @@ def cpu_baseline(n, phiflag, fields, dt, steps=2):
                       f"{steps} steps, the last one timed, {threads} OpenMP threads"}
 
 
+def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
+    """Measured stream-copy ceiling: a plain 1 GiB device-to-device copy
+    (read + write bytes over its event-timed duration)."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def ensemble_leg(count=16, steps=(200, 800), inflight=2):
+    """The serving stream: `count` seeded Poisson-arrival requests (40/s) of
+    independent boxes, N in {64, 96, 128, 192, 256}, nu in [0.002, 0.02],
+    half with ACDI drops, mixed precision.  Run one box at a time (create,
+    step, copy out, destroy) and through Ensemble (submitted at their arrival
+    times, continuous admission), compare results, report throughput in cell
+    updates per second and the step latency percentiles."""
+    import numpy as np
+    from mhit36_amd import serve
+    reqs = serve.c5_stream(count=count, steps=steps)
+    cells = serve.cell_steps(reqs)
+    t_one, out_one = serve.run_one_at_a_time(reqs)
+    wall, out_ens, lat, res = serve.run_ensemble(reqs, follow_arrivals=True, inflight=inflight)
+    same = all(np.array_equal(out_one[q], out_ens[q]) for q in out_one)
+    return {"workload": f"{count} requests, N in {list(serve.SIZES)}, {steps[0]}-{steps[1]} steps, "
+                        "half ACDI, mixed precision, Poisson arrivals 40/s",
+            "cell_steps": cells, "one_at_a_time_s": round(t_one, 4), "ensemble_s": round(wall, 4),
+            "throughput_gcell_steps_s": round(cells / wall / 1e9, 3),
+            "one_at_a_time_gcell_steps_s": round(cells / t_one / 1e9, 3),
+            "speedup": round(t_one / wall, 3), "step_latency_p50_ms": round(lat["p50_ms"], 4),
+            "step_latency_p99_ms": round(lat["p99_ms"], 4), "steps_timed": lat["steps"],
+            "same_alone_and_in_mix": bool(same), "status": sorted(set(r["status"] for r in res)),
+            "inflight": inflight}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +142,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving leg")
+    ap.add_argument("--no-autotune", action="store_true", help="heuristic launch shapes only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (3 times), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -119,6 +166,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 8}), flush=True)
         return
+    copy_gbs = stream_copy_gbs(torch)
     n, phiflag = args.n, args.phiflag
     t_ic = time.perf_counter()
     u, v, w, phi, dt = make_inputs(n, phiflag)
@@ -128,8 +176,12 @@ def main():
     print(f"[bench] inputs {n}^3 ready in {time.perf_counter() - t_ic:.1f}s, dt={dt:.4e}",
           file=sys.stderr, flush=True)
 
+    tunings = {}
+
     def timed_run(precision, kernel_timing):
         s = Solver(**params_for(n, phiflag, dt), precision=precision)
+        if not args.no_autotune:
+            tunings["fp64" if precision == 0 else "mixed"] = s.autotune()
         for f, a in fields.items():
             s.set_field(f, a)
         s.step(args.warmup)
@@ -174,6 +226,12 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, phiflag, dom["name"]),
                 "kernel": dom["name"], "avg_ms": round(avg, 4), "alg_bytes_per_launch": dom["bytes"]}
 
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg()
+        print(f"[bench] ensemble {ens['ensemble_s']:.3f}s vs one at a time {ens['one_at_a_time_s']:.3f}s, "
+              f"p99 {ens['step_latency_p99_ms']:.3f} ms", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, phiflag, fields, dt)
@@ -199,9 +257,12 @@ def main():
         "step_roofline_frac": round(balg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "alg_bytes_per_step": balg,
         "device_bytes": dev_bytes,
+        "stream_copy_gbs": copy_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "tuning": {k: v["value"] for k, v in tunings.items()},
+        "ensemble": ens,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MHIT_ABI_VERSION 1
+#define MHIT_ABI_VERSION 2
 
 enum {
     MHIT_OK = 0,
@@ -158,6 +158,32 @@ int mhit_kernel_stats(mhit_ctx *ctx, mhit_kstat *out, int max_entries, int *n_en
 /* Device memory held by the context, bytes. */
 int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose kernel
+ * shapes are fixed by nvfortran): times the candidate launch shapes of the
+ * tunable passes -- z planes per block of the phase-field and momentum passes,
+ * line pairs per block of the x FFT passes, spectrum columns per block of the
+ * y and z passes -- on this context and keeps the fastest.  The shapes split
+ * work differently, never change the arithmetic, so results are bit-identical
+ * for every shape.  Choices are cached per (nx, precision, phiflag) for the
+ * process: later contexts with the same key start with them.  The passes run
+ * on the context's buffers, so the call is allowed only on a context whose
+ * fields were not yet set (MHIT_ERR_ARG otherwise); it leaves every field
+ * zero.  Without MHIT_TUNE_FORCE a cached choice is reused without timing. */
+#define MHIT_TUNE_FORCE 1
+enum { MHIT_KNOB_KZ_PHASE = 0, MHIT_KNOB_KZ_MOMENTUM, MHIT_KNOB_X_PAIRS, MHIT_KNOB_XINV_PAIRS, MHIT_KNOB_Y_TILE,
+       MHIT_KNOB_Z_TILE, MHIT_KNOBS };
+typedef struct mhit_tuning {
+    int value[MHIT_KNOBS];        /* chosen shape of each knob                           */
+    int heuristic[MHIT_KNOBS];    /* the built-in heuristic's shape                      */
+    double ms_heuristic;          /* sum of the tuned passes' times, heuristic shapes    */
+    double ms_tuned;              /* the same with the chosen shapes (0 if not timed)    */
+} mhit_tuning;
+int mhit_autotune(mhit_ctx *ctx, int flags, mhit_tuning *out);
+int mhit_get_tuning(mhit_ctx *ctx, mhit_tuning *out);
+/* Set this context's shapes by hand: value[k] = 0 selects the heuristic, any
+ * other value must be one of the knob's valid shapes (MHIT_ERR_ARG if not). */
+int mhit_set_tuning(mhit_ctx *ctx, const mhit_tuning *in);
+
 /* Diagnostics (no counterpart in the reference): evaluate the device's own
  * elementary functions on `count` arguments (op 0 log, 1 tanh, 2 sqrt,
  * 3 reciprocal), and copy an intermediate device buffer of the last step
@@ -178,7 +204,7 @@ typedef struct mhit_ens_config {
     int max_active;           /* boxes resident at once (0: 64)                          */
     int inflight;             /* steps queued ahead per box (0: 2)                       */
     int use_graphs;           /* 1: replay each box's step as a captured hipGraph        */
-    int reserved;
+    int autotune;             /* 1: autotune each new (nx, precision, phiflag) at admission */
 } mhit_ens_config;
 
 typedef struct mhit_ens_result {
@@ -205,7 +231,8 @@ int mhit_ens_results(mhit_ens *ens, mhit_ens_result *out, int max_entries, int *
 /* Final interior field (fieldn 1..5) of a finished request, kept on the host. */
 int mhit_ens_get_field(mhit_ens *ens, long long id, int fieldn, double *host);
 int mhit_ens_release(mhit_ens *ens, long long id);
-/* Per-step device latency (stream events around each step) of all steps so far. */
+/* Step latency: the device-clock interval between consecutive step
+ * completions of one box (every step after each box's first), all boxes. */
 int mhit_ens_latency(mhit_ens *ens, double *p50_ms, double *p99_ms, double *mean_ms, long long *count);
 
 const char *mhit_strerror(int code);

@@ -31,7 +31,7 @@ EXPORTS = (
     "mhit_destroy", "mhit_init_fields", "mhit_set_field", "mhit_get_field",
     "mhit_step", "mhit_reset_time", "mhit_write_field", "mhit_read_field",
     "mhit_diagnostics", "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
-    "mhit_debug_math", "mhit_debug_buffer",
+    "mhit_debug_math", "mhit_debug_buffer", "mhit_autotune", "mhit_get_tuning", "mhit_set_tuning",
     "mhit_ens_create", "mhit_ens_destroy", "mhit_ens_submit", "mhit_ens_progress", "mhit_ens_wait_idle",
     "mhit_ens_results", "mhit_ens_get_field", "mhit_ens_release", "mhit_ens_latency",
     "mhit_strerror", "mhit_abi_version",
@@ -82,6 +82,20 @@ class KStat(ctypes.Structure):
     ]
 
 
+KNOBS = ("kz_phase", "kz_momentum", "x_pairs", "xinv_pairs", "y_tile", "z_tile")
+MHIT_TUNE_FORCE = 1
+
+
+class Tuning(ctypes.Structure):
+    """mhit_tuning: launch shape of each tunable pass."""
+    _fields_ = [("value", ctypes.c_int * len(KNOBS)), ("heuristic", ctypes.c_int * len(KNOBS)),
+                ("ms_heuristic", ctypes.c_double), ("ms_tuned", ctypes.c_double)]
+
+    def to_dict(self):
+        return {"value": dict(zip(KNOBS, self.value)), "heuristic": dict(zip(KNOBS, self.heuristic)),
+                "ms_heuristic": self.ms_heuristic, "ms_tuned": self.ms_tuned}
+
+
 class MhitError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -122,6 +136,9 @@ def load():
         "mhit_device_bytes": ([vp, P(ctypes.c_size_t)], ctypes.c_int),
         "mhit_debug_math": ([ctypes.c_int, ctypes.c_size_t, vp, vp], ctypes.c_int),
         "mhit_debug_buffer": ([vp, ctypes.c_int, vp], ctypes.c_int),
+        "mhit_autotune": ([vp, ctypes.c_int, P(Tuning)], ctypes.c_int),
+        "mhit_get_tuning": ([vp, P(Tuning)], ctypes.c_int),
+        "mhit_set_tuning": ([vp, P(Tuning)], ctypes.c_int),
         "mhit_strerror": ([ctypes.c_int], ctypes.c_char_p),
         "mhit_abi_version": ([], ctypes.c_int),
     }

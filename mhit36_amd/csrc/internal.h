@@ -81,15 +81,27 @@ struct LaunchInfo {
     double bytes;
 };
 
-void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta,
+// Launch shape of each tunable pass; 0 selects the built-in heuristic.  The
+// shape changes only how work is split over blocks, never the arithmetic.
+enum Knob { KNOB_KZ_ACDI, KNOB_KZ_MOM, KNOB_XPAIRS, KNOB_XIPAIRS, KNOB_YTILE, KNOB_ZTILE, KNOB_COUNT };
+struct Tuning {
+    int v[KNOB_COUNT] = {0, 0, 0, 0, 0, 0};
+};
+// Valid values of `knob` for this box (at most `cap`), and the heuristic's value.
+int knob_candidates(int knob, const Consts &c, const FftPlan &pl, int *out, int cap);
+int knob_default(int knob, const Consts &c, const FftPlan &pl);
+
+void launch_acdi(const Fields &f, const Consts &c, const Tuning &tu, hipStream_t s, LaunchInfo *li);
+void launch_momentum(const Fields &f, const Consts &c, const Tuning &tu, double alpha, double beta,
                      hipStream_t s, LaunchInfo *li);
-void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li);
-void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse,
+void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
+                 LaunchInfo *li);
+void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, int inverse,
                   hipStream_t s, LaunchInfo *li);
-void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s,
+void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
                      LaunchInfo *li);
-void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li);
+void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
+                 LaunchInfo *li);
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);

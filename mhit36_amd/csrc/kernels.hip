@@ -378,9 +378,9 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         if (k + 1 < kend) fetch_u(k + 2);
         const T ro0 = ro[0], ro1 = ro[1], ro2 = ro[2];
         if (active && k + 1 < kend) {
-            ro[0] = ruo[cxy + nn * (size_t)(k + 1)];
-            ro[1] = rvo[cxy + nn * (size_t)(k + 1)];
-            ro[2] = rwo[cxy + nn * (size_t)(k + 1)];
+            ro[0] = __builtin_nontemporal_load(&ruo[cxy + nn * (size_t)(k + 1)]);
+            ro[1] = __builtin_nontemporal_load(&rvo[cxy + nn * (size_t)(k + 1)]);
+            ro[2] = __builtin_nontemporal_load(&rwo[cxy + nn * (size_t)(k + 1)]);
         }
         __syncthreads();
         if (active) {
@@ -433,12 +433,12 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
             }
             const T dt = (T)c.dt, alpha = (T)ALPHA, beta = (T)BETA;
             const size_t t = cxy + nn * (size_t)k;
-            us[t] = u0 + dt * (alpha * ru - beta * ro0);
-            vs[t] = v0 + dt * (alpha * rv - beta * ro1);
-            ws[t] = w0 + dt * (alpha * rw - beta * ro2);
-            ruo[t] = ru;
-            rvo[t] = rv;
-            rwo[t] = rw;
+            __builtin_nontemporal_store(u0 + dt * (alpha * ru - beta * ro0), &us[t]);
+            __builtin_nontemporal_store(v0 + dt * (alpha * rv - beta * ro1), &vs[t]);
+            __builtin_nontemporal_store(w0 + dt * (alpha * rw - beta * ro2), &ws[t]);
+            __builtin_nontemporal_store(ru, &ruo[t]);
+            __builtin_nontemporal_store(rv, &rvo[t]);
+            __builtin_nontemporal_store(rw, &rwo[t]);
         }
     }
 }
@@ -1177,9 +1177,9 @@ static int march_kz(int n, int tx, int ty)
     return 1;
 }
 
-void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
+void launch_acdi(const Fields &f, const Consts &c, const Tuning &tu, hipStream_t s, LaunchInfo *li)
 {
-    const int n = c.n, kz = march_kz(n, ATX, ATY);
+    const int n = c.n, kz = tu.v[KNOB_KZ_ACDI] ? tu.v[KNOB_KZ_ACDI] : march_kz(n, ATX, ATY);
     dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, n / kz);
     if (c.mixed)
         hipLaunchKernelGGL(k_acdi<float>, grid, dim3(ATHREADS), 0, s, (const float *)f.u, (const float *)f.v,
@@ -1189,10 +1189,10 @@ void launch_acdi(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li
     li->name = "acdi"; li->bytes = (c.mixed ? 2.5 : 5.0) * F(c);
 }
 
-void launch_momentum(const Fields &f, const Consts &c, double alpha, double beta, hipStream_t s,
+void launch_momentum(const Fields &f, const Consts &c, const Tuning &tu, double alpha, double beta, hipStream_t s,
                      LaunchInfo *li)
 {
-    const int n = c.n, kz = march_kz(n, MTX, MTY);
+    const int n = c.n, kz = tu.v[KNOB_KZ_MOM] ? tu.v[KNOB_KZ_MOM] : march_kz(n, MTX, MTY);
     dim3 grid((n + MTX - 1) / MTX, (n + MTY - 1) / MTY, n / kz);
     if (c.mixed) {
         using T = float;
@@ -1281,10 +1281,48 @@ static int fft_threads(const FftPlan &pl, int nlines)
     return t < 64 ? 64 : t;
 }
 
-void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+// Tunable launch shapes.  Every candidate satisfies what the kernels assume:
+// z planes per block divide n; blocks have at most FFT_MAX_THREADS threads and
+// LDS_CAP bytes of LDS; pairs per block divide the n^2/2 line pairs.
+int knob_default(int knob, const Consts &c, const FftPlan &pl)
+{
+    const bool regs = plan_regs(pl);
+    switch (knob) {
+    case KNOB_KZ_ACDI: return march_kz(c.n, ATX, ATY);
+    case KNOB_KZ_MOM: return march_kz(c.n, MTX, MTY);
+    case KNOB_XPAIRS:
+    case KNOB_XIPAIRS: return regs ? rpairs(pl) : pairs_per_block(pl);
+    default: return regs ? rtile(pl) : col_tile(pl);
+    }
+}
+
+int knob_candidates(int knob, const Consts &c, const FftPlan &pl, int *out, int cap)
+{
+    const int n = c.n;
+    const bool regs = plan_regs(pl);
+    const int per_line = regs ? n / 8 : n / plan_rmin(pl);   // threads per line
+    int k = 0;
+    for (int v = 1; v <= 64 && k < cap; v *= 2) {
+        bool ok;
+        if (knob == KNOB_KZ_ACDI || knob == KNOB_KZ_MOM) {
+            ok = v >= 2 && v <= n && n % v == 0;
+        } else if (knob == KNOB_XPAIRS || knob == KNOB_XIPAIRS) {
+            const size_t lds = sizeof(double2) * (size_t)v * (regs ? rline_stride(n) : n + 1);
+            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_CAP && (n * n / 2) % v == 0;
+        } else {
+            const size_t lds = sizeof(double2) * (size_t)v * n;
+            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_CAP && v <= n / 2 + 1;
+        }
+        if (ok) out[k++] = v;
+    }
+    return k;
+}
+
+void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
+                 LaunchInfo *li)
 {
     if (plan_regs(pl)) {
-        const int ppb = rpairs(pl), pairs = c.n * c.n / 2;
+        const int ppb = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
         const dim3 grid(pairs / ppb), blk(ppb * (c.n / 8));
         if (c.mixed)
@@ -1296,7 +1334,7 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
         li->name = "xfwd_div"; li->bytes = (c.mixed ? 2.5 : 4.0) * F(c);
         return;
     }
-    int ppb = pairs_per_block(pl);
+    int ppb = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
     if (c.mixed)
@@ -1308,11 +1346,11 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_
     li->name = "xfwd_div"; li->bytes = (c.mixed ? 2.5 : 4.0) * F(c);
 }
 
-void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inverse, hipStream_t s,
-                  LaunchInfo *li)
+void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, int inverse,
+                  hipStream_t s, LaunchInfo *li)
 {
     if (plan_regs(pl)) {
-        const int tkx = rtile(pl), nh = c.n / 2 + 1;
+        const int tkx = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl), nh = c.n / 2 + 1;
         const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
         const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
         if (inverse)
@@ -1323,7 +1361,7 @@ void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inver
         li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
         return;
     }
-    int tkx = col_tile(pl);
+    int tkx = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : col_tile(pl);
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
     size_t shm = sizeof(double2) * (size_t)tkx * c.n;
@@ -1336,10 +1374,11 @@ void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, int inver
     li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
 }
 
-void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
+                     LaunchInfo *li)
 {
     if (plan_regs(pl)) {
-        const int tkx = rtile(pl), nh = c.n / 2 + 1;
+        const int tkx = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl), nh = c.n / 2 + 1;
         const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
         const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
         hipLaunchKernelGGL(k_zpoisson_r, grid, blk, shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
@@ -1347,7 +1386,7 @@ void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStr
         li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
         return;
     }
-    int tkx = col_tile(pl);
+    int tkx = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
     int nh = c.n / 2 + 1;
     dim3 grid((nh + tkx - 1) / tkx, c.n);
     size_t shm = sizeof(double2) * (size_t)tkx * c.n;
@@ -1357,17 +1396,18 @@ void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, hipStr
     li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
 }
 
-void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, hipStream_t s, LaunchInfo *li)
+void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, hipStream_t s,
+                 LaunchInfo *li)
 {
     if (plan_regs(pl)) {
-        const int ppb = rpairs(pl), pairs = c.n * c.n / 2;
+        const int ppb = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
         hipLaunchKernelGGL(k_xinv_r, dim3(pairs / ppb), dim3(ppb * (c.n / 8)), shm, s, f.spec, f.p, f.tw, f.sc, pl,
                            ppb, f.ld);
         li->name = "x_inv"; li->bytes = 16.0 * (double)(c.n / 2 + 1) * c.n * c.n + F(c);
         return;
     }
-    int ppb = pairs_per_block(pl);
+    int ppb = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : pairs_per_block(pl);
     int pairs = c.n * c.n / 2;
     size_t shm = sizeof(double2) * (size_t)ppb * (c.n + 1);
     hipLaunchKernelGGL(k_xinv, dim3(pairs / ppb), dim3(fft_threads(pl, ppb)), shm, s, f.spec, f.p, f.tw, f.sc, pl,

@@ -100,6 +100,8 @@ struct mhit_ctx {
     Arena *arena = nullptr;      // sub-allocate from here instead of hipMalloc
     bool own_stream = true;
     std::vector<double> x;       // grid, main.f90:149-153
+    Tuning tune;                 // launch shapes (0: heuristic), see mhit_autotune
+    bool loaded = false;         // a field was set since creation
     bool timing = false;
     std::vector<hipEvent_t> event_pool;
     size_t event_next = 0;
@@ -304,6 +306,7 @@ extern "C" int mhit_destroy(mhit_ctx *ctx)
 }
 
 static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipStream_t stream);
+static void cached_tuning(mhit_ctx *c);
 
 extern "C" int mhit_create(const mhit_params *p, mhit_ctx **out)
 {
@@ -412,6 +415,7 @@ static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipSt
         mhit_destroy(c);
         return MHIT_ERR_HIP;
     }
+    cached_tuning(c);
     *out = c;
     return MHIT_OK;
 }
@@ -442,6 +446,7 @@ extern "C" int mhit_set_field(mhit_ctx *c, int fieldn, const double *host)
     if (!c || !host) return MHIT_ERR_ARG;
     double *d = field_ptr(c, fieldn);
     if (!d) return MHIT_ERR_ARG;
+    c->loaded = true;
     const size_t n = c->p.nx, n3 = n * n * n;
     std::vector<float> tmp;
     if (c->kc.mixed && fieldn != 4) {
@@ -622,15 +627,15 @@ static void enqueue_step(mhit_ctx *c)
     const double alpha = c->nsteps_since_init == 0 ? 1.0 : 1.5;
     const double beta = c->nsteps_since_init == 0 ? 0.0 : 0.5;
     if (k.phiflag) {
-        timed(c, [&](LaunchInfo *li) { launch_acdi(f, k, s, li); });
+        timed(c, [&](LaunchInfo *li) { launch_acdi(f, k, c->tune, s, li); });
         std::swap(f.phi, f.phin);
     }
-    timed(c, [&](LaunchInfo *li) { launch_momentum(f, k, alpha, beta, s, li); });
-    timed(c, [&](LaunchInfo *li) { launch_xfwd(f, k, c->plan, s, li); });
-    timed(c, [&](LaunchInfo *li) { launch_ypass(f, k, c->plan, 0, s, li); });
-    timed(c, [&](LaunchInfo *li) { launch_zpoisson(f, k, c->plan, s, li); });
-    timed(c, [&](LaunchInfo *li) { launch_ypass(f, k, c->plan, 1, s, li); });
-    timed(c, [&](LaunchInfo *li) { launch_xinv(f, k, c->plan, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_momentum(f, k, c->tune, alpha, beta, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_xfwd(f, k, c->plan, c->tune, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_ypass(f, k, c->plan, c->tune, 0, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_zpoisson(f, k, c->plan, c->tune, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_ypass(f, k, c->plan, c->tune, 1, s, li); });
+    timed(c, [&](LaunchInfo *li) { launch_xinv(f, k, c->plan, c->tune, s, li); });
     timed(c, [&](LaunchInfo *li) { launch_project(f, k, s, li); });
     timed(c, [&](LaunchInfo *li) { launch_reduce(f, k, s, li); });
     timed(c, [&](LaunchInfo *li) { launch_finalize(f, k, s, li); });
@@ -715,6 +720,172 @@ extern "C" int mhit_read_field(mhit_ctx *c, const char *path, int fieldn)
     fclose(fp);
     if (got != n3) return MHIT_ERR_IO;
     return mhit_set_field(c, fieldn, a.data());
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+    int n, mixed, phi;
+    bool operator<(const TuneKey &o) const
+    {
+        return n != o.n ? n < o.n : (mixed != o.mixed ? mixed < o.mixed : phi < o.phi);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tuning> g_tune_cache;   // process-wide, filled by mhit_autotune
+}  // namespace
+
+static TuneKey tune_key(const mhit_ctx *c) { return {c->kc.n, c->kc.mixed, c->kc.phiflag}; }
+
+static void cached_tuning(mhit_ctx *c)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find(tune_key(c));
+    if (it != g_tune_cache.end()) c->tune = it->second;
+}
+
+// The pass that `knob` shapes, once, on the context's buffers, with the
+// constants of a first (Euler) step.
+static void tune_launch(mhit_ctx *c, int knob, const Tuning &tu)
+{
+    LaunchInfo li;
+    const Fields &f = c->f;
+    const Consts &k = c->kc;
+    hipStream_t s = c->stream;
+    switch (knob) {
+    case KNOB_KZ_ACDI: launch_acdi(f, k, tu, s, &li); break;
+    case KNOB_KZ_MOM: launch_momentum(f, k, tu, 1.0, 0.0, s, &li); break;
+    case KNOB_XPAIRS: launch_xfwd(f, k, c->plan, tu, s, &li); break;
+    case KNOB_XIPAIRS: launch_xinv(f, k, c->plan, tu, s, &li); break;
+    case KNOB_YTILE: launch_ypass(f, k, c->plan, tu, 0, s, &li); break;
+    default: launch_zpoisson(f, k, c->plan, tu, s, &li); break;
+    }
+}
+
+// Mean milliseconds of `reps` launches after one untimed launch.
+static int tune_time(mhit_ctx *c, int knob, const Tuning &tu, hipEvent_t a, hipEvent_t b, int reps, double *ms)
+{
+    tune_launch(c, knob, tu);
+    HIPCHK(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) tune_launch(c, knob, tu);
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = (double)t / reps;
+    return MHIT_OK;
+}
+
+static void fill_tuning(const mhit_ctx *c, mhit_tuning *out, double ms_h, double ms_t)
+{
+    if (!out) return;
+    memset(out, 0, sizeof(*out));
+    for (int q = 0; q < KNOB_COUNT; q++) {
+        out->heuristic[q] = knob_default(q, c->kc, c->plan);
+        out->value[q] = c->tune.v[q] ? c->tune.v[q] : out->heuristic[q];
+    }
+    out->ms_heuristic = ms_h;
+    out->ms_tuned = ms_t;
+}
+
+static int tune_impl(mhit_ctx *c, double *ms_h, double *ms_t)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return MHIT_ERR_HIP;
+    }
+    const int reps = c->kc.n >= 256 ? 5 : 20;
+    Tuning best;
+    int rc = MHIT_OK;
+    for (int knob = 0; knob < KNOB_COUNT && rc == MHIT_OK; knob++) {
+        if (knob == KNOB_KZ_ACDI && !c->kc.phiflag) continue;
+        int cand[8];
+        const int nc = knob_candidates(knob, c->kc, c->plan, cand, 8);
+        const int def = knob_default(knob, c->kc, c->plan);
+        Tuning t = best;
+        t.v[knob] = def;
+        double def_ms = 0.0;
+        if ((rc = tune_time(c, knob, t, a, b, reps, &def_ms))) break;
+        int pick = def;
+        double pick_ms = def_ms;
+        for (int q = 0; q < nc && rc == MHIT_OK; q++) {
+            if (cand[q] == def) continue;
+            t.v[knob] = cand[q];
+            double ms = 0.0;
+            if ((rc = tune_time(c, knob, t, a, b, reps, &ms))) break;
+            if (ms < pick_ms) { pick = cand[q]; pick_ms = ms; }
+        }
+        // keep the heuristic unless the gain is clear of timing noise
+        if (pick != def && pick_ms > 0.97 * def_ms) { pick = def; pick_ms = def_ms; }
+        best.v[knob] = pick;
+        *ms_h += def_ms;
+        *ms_t += pick_ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    // the passes wrote u*, rhs_o, phi^{n+1}, the spectrum and p: back to zero
+    const size_t n3 = (size_t)c->kc.n * c->kc.n * c->kc.n;
+    const size_t FT = n3 * (c->kc.mixed ? sizeof(float) : sizeof(double));
+    double *st[] = {c->f.us, c->f.vs, c->f.ws, c->f.ruo, c->f.rvo, c->f.rwo};
+    for (double *q : st) HIPCHK(hipMemsetAsync(q, 0, FT, c->stream));
+    if (c->kc.phiflag) HIPCHK(hipMemsetAsync(c->f.phin, 0, FT, c->stream));
+    HIPCHK(hipMemsetAsync(c->f.p, 0, n3 * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->f.spec, 0, sizeof(double2) * (size_t)c->kc.n * c->kc.n * c->f.ld, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->tune = best;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[tune_key(c)] = best;
+    return MHIT_OK;
+}
+
+extern "C" int mhit_autotune(mhit_ctx *c, int flags, mhit_tuning *out)
+{
+    if (!c) return MHIT_ERR_ARG;
+    if (!(flags & MHIT_TUNE_FORCE)) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(tune_key(c));
+        if (it != g_tune_cache.end()) {
+            c->tune = it->second;
+            fill_tuning(c, out, 0.0, 0.0);
+            return MHIT_OK;
+        }
+    }
+    if (c->loaded || c->nsteps_since_init != 0 || c->halted) return MHIT_ERR_ARG;
+    double ms_h = 0.0, ms_t = 0.0;
+    int rc = tune_impl(c, &ms_h, &ms_t);
+    if (rc) return rc;
+    fill_tuning(c, out, ms_h, ms_t);
+    return MHIT_OK;
+}
+
+extern "C" int mhit_set_tuning(mhit_ctx *c, const mhit_tuning *in)
+{
+    if (!c || !in) return MHIT_ERR_ARG;
+    Tuning t;
+    for (int q = 0; q < KNOB_COUNT; q++) {
+        const int v = in->value[q];
+        if (v == 0) continue;
+        int cand[8];
+        const int nc = knob_candidates(q, c->kc, c->plan, cand, 8);
+        bool ok = false;
+        for (int i = 0; i < nc; i++) ok = ok || cand[i] == v;
+        if (!ok) return MHIT_ERR_ARG;
+        t.v[q] = v;
+    }
+    c->tune = t;
+    return MHIT_OK;
+}
+
+extern "C" int mhit_get_tuning(mhit_ctx *c, mhit_tuning *out)
+{
+    if (!c || !out) return MHIT_ERR_ARG;
+    fill_tuning(c, out, 0.0, 0.0);
+    return MHIT_OK;
 }
 
 extern "C" int mhit_diagnostics(mhit_ctx *c, double dxi, double nu, mhit_stats *out)
@@ -947,7 +1118,8 @@ static int io_admit(mhit_ens *e, EnsReq *r)
     int rc = create_impl(&r->p, &r->ctx, &e->arena, r->stream);
     if (rc) return rc;
     mhit_ctx *c = r->ctx;
-    if (r->in[0].empty() || (r->p.phiflag && r->in[3].empty())) rc = mhit_init_fields(c);
+    if (e->cfg.autotune) rc = mhit_autotune(c, 0, nullptr);   // first box of its kind: timed on the fresh box
+    if (rc == MHIT_OK && (r->in[0].empty() || (r->p.phiflag && r->in[3].empty()))) rc = mhit_init_fields(c);
     const int fids[4] = {1, 2, 3, 5};
     for (int q = 0; q < 4 && rc == MHIT_OK; q++)
         if (!r->in[q].empty() && (q < 3 || r->p.phiflag)) rc = mhit_set_field(c, fids[q], r->in[q].data());

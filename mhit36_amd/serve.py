@@ -45,12 +45,15 @@ def cell_steps(reqs):
     return float(sum(r["params"]["nx"] ** 3 * r["nsteps"] for r in reqs))
 
 
-def run_one_at_a_time(reqs):
-    """Each box alone: create, run all its steps, copy out, destroy."""
+def run_one_at_a_time(reqs, autotune=True):
+    """Each box alone: create, (launch shapes from the autotuner's cache, tuned
+    on the first box of each size), run all its steps, copy out, destroy."""
     t0 = time.perf_counter()
     out = {}
     for q, r in enumerate(reqs):
         with Solver(**r["params"]) as s:
+            if autotune:
+                s.autotune()
             s.init_fields()
             if r["phi"] is not None:
                 s.set_field(5, r["phi"])

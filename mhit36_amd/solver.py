@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import Params, Consts, Diag, KStat, Stats, check, MhitError
+from ._lib import Params, Consts, Diag, KStat, Stats, Tuning, MHIT_TUNE_FORCE, check, MhitError
 
 FIELD_NAMES = {1: "u", 2: "v", 3: "w", 4: "p", 5: "phi"}   # readwrite.f90:21-25
 
@@ -144,6 +144,27 @@ class Solver:
         nu = self.params.mu / self.params.rho if nu is None else nu
         check(self._lib.mhit_diagnostics(self._h, float(dxi), float(nu), ctypes.byref(st)), "diagnostics")
         return {name: getattr(st, name) for name, _ in Stats._fields_}
+
+    # -- launch shapes -----------------------------------------------------------
+    def autotune(self, force=False):
+        """Time the candidate launch shapes of the tunable passes on this fresh
+        box (before any field is set) and keep the fastest; cached per
+        (nx, precision, phiflag) for the process (see mhit_autotune)."""
+        t = Tuning()
+        check(self._lib.mhit_autotune(self._h, MHIT_TUNE_FORCE if force else 0, ctypes.byref(t)), "autotune")
+        return t.to_dict()
+
+    def tuning(self):
+        t = Tuning()
+        check(self._lib.mhit_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return t.to_dict()
+
+    def set_tuning(self, **shapes):
+        """Set launch shapes by knob name (0 = heuristic); see _lib.KNOBS."""
+        t = Tuning()
+        for k, v in shapes.items():
+            t.value[_lib.KNOBS.index(k)] = int(v)
+        check(self._lib.mhit_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
 
     # -- measurement ------------------------------------------------------------
     def kernel_timing(self, enable=True):
