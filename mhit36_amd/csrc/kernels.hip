@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
             T Y = (sxyL[sxy][1][sc0 + ASX] - sxyL[sxy][1][sc0 - ASX]) * (T)0.5 * dxi;
             T Z = (szL[sp][sc0] - szL[sm][sc0]) * (T)0.5 * dxi;
             T r2 = rphi - gamma * ((X + Y) + Z);
-            phin[cxy + nn * (size_t)k] = f0 + (T)c.dt * r2;
+            __builtin_nontemporal_store(f0 + (T)c.dt * r2, &phin[cxy + nn * (size_t)k]);
         }
     }
 }
@@ -754,6 +754,14 @@ __device__ void fft_regs(double2 (&v)[8], int tj, int line, double2 *lds, int ls
 
 __host__ __device__ constexpr int rline_stride(int n) { return n + n / 8 + 1; }
 
+// 16-byte streaming store of one complex value (written once, not re-read by this kernel).
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void nt_store2(double2 *dst, double re, double im)
+{
+    nt_double2 v = {re, im};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_double2 *>(dst));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
@@ -799,8 +807,8 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
         int mq = q == 0 ? 0 : n - q;
         double2 zq = lds[rlds<true>(pq, q, lstride, 1)], zm = lds[rlds<true>(pq, mq, lstride, 1)];
         double a = zq.x, b = zq.y, cc = zm.x, d = zm.y;
-        spec[((size_t)kk * n + jj) * ld + q] = make_double2((a + cc) * 0.5, (b - d) * 0.5);
-        spec[((size_t)kk * n + jj + 1) * ld + q] = make_double2((b + d) * 0.5, (cc - a) * 0.5);
+        nt_store2(&spec[((size_t)kk * n + jj) * ld + q], (a + cc) * 0.5, (b - d) * 0.5);
+        nt_store2(&spec[((size_t)kk * n + jj + 1) * ld + q], (b + d) * 0.5, (cc - a) * 0.5);
     }
 }
 
@@ -823,7 +831,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict
     fft_regs<SG, false>(v, tj, line, lds, 1, tkx, pl, tw);
     if (ok) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) base[(size_t)(tj + m * LM) * ld] = v[m];
+        for (int m = 0; m < 8; m++) nt_store2(&base[(size_t)(tj + m * LM) * ld], v[m].x, v[m].y);
     }
 }
 
@@ -861,7 +869,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restr
     fft_regs<1, false>(v, tj, line, lds, 1, tkx, pl, tw);
     if (ok) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) base[(size_t)(tj + m * LM) * rs] = v[m];
+        for (int m = 0; m < 8; m++) nt_store2(&base[(size_t)(tj + m * LM) * rs], v[m].x, v[m].y);
     }
 }
 
@@ -900,8 +908,8 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__res
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int i = tj + m * LM;
-        p[IDX(i, j, k)] = v[m].x;
-        p[IDX(i, j + 1, k)] = v[m].y;
+        __builtin_nontemporal_store(v[m].x, &p[IDX(i, j, k)]);
+        __builtin_nontemporal_store(v[m].y, &p[IDX(i, j + 1, k)]);
     }
 }
 
@@ -965,9 +973,9 @@ __global__ void __launch_bounds__(BLK) k_project(
         const T tu = (T)((double)us[id] - dtr * (p0 - p[IDX(im, j, k)]) * dxi);
         const T tv = (T)((double)vs[id] - dtr * (p0 - p[IDX(i, jm, k)]) * dxi);
         const T tw = (T)((double)ws[id] - dtr * (p0 - p[IDX(i, j, km)]) * dxi);
-        u[id] = tu;
-        v[id] = tv;
-        w[id] = tw;
+        __builtin_nontemporal_store(tu, &u[id]);
+        __builtin_nontemporal_store(tv, &v[id]);
+        __builtin_nontemporal_store(tw, &w[id]);
         const double uu = tu, vv = tv, ww = tw;
         if constexpr (sizeof(T) == 4) {
             neumaier(su, cu, uu); neumaier(sv, cv, vv); neumaier(sw, cw, ww);
