@@ -15,71 +15,75 @@ namespace mhit {
 constexpr double LN2_HI = 6.93147180369123816490e-01;
 constexpr double LN2_LO = 1.90821492927058770002e-10;
 
-// Natural logarithm.  x = 2^e m, m in [sqrt(1/2), sqrt(2)), f = m - 1,
-// s = f/(2+f); log(1+f) = 2 atanh(s) = 2s + s R(s^2) with the Taylor series
-// R(z) = sum_{k=1..12} 2/(2k+1) z^k, evaluated as f - (f^2/2 - s(f^2/2 + R)).
-__device__ __forceinline__ double dlog(double x)
+// Table of the table-driven log and expm1 (built on the host by
+// build_math_table, identical to the oracle's):
+//   [TAB_INVC + i]  1/c_i rounded to 8 significant bits, c_i = 1 + (i+1/2)/128,
+//                   i < 128; entry 128 is 1 (the |x-1| < 2^-7 path);
+//   [TAB_LHI + i], [TAB_LLO + i]   -log(invc_i) as a double-double;
+//   [TAB_EHI + j], [TAB_ELO + j]   2^(j/32) as a double-double, j < 32.
+constexpr int TAB_INVC = 0, TAB_LHI = 129, TAB_LLO = 258, TAB_EHI = 387, TAB_ELO = 419, MATH_TAB = 451;
+
+// Natural logarithm.  x = 2^e m, m in [1,2); i = the top 7 fraction bits of
+// m; r = m*invc_i - 1 is exact (invc_i has 8 significant bits, |r| <= 2^-7)
+// and one fma forms it; log x = e ln2 - log(invc_i) + log1p(r), log1p by its
+// Taylor series through r^8.  Within 2^-7 of 1, r = x - 1 (exact) and the
+// table term is 0.  The leading sum is formed error-free (Knuth's TwoSum).
+__device__ __forceinline__ double dlog(double x, const double *__restrict__ tab)
 {
     if (!(x > 0.0)) return (x == 0.0) ? -__builtin_inf() : __builtin_nan("");
     if (__builtin_isinf(x)) return x;
     int e = 0;
     if (x < 2.2250738585072014e-308) { x = x * 18014398509481984.0; e = -54; }
-    uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
     e += (int)((b >> 52) & 0x7ff) - 1023;
     double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
-    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
-    double f = m - 1.0;
-    double s = f / (2.0 + f);
-    double z = s * s;
-    double R = 2.0 / 25.0;
-    R = 2.0 / 23.0 + z * R;
-    R = 2.0 / 21.0 + z * R;
-    R = 2.0 / 19.0 + z * R;
-    R = 2.0 / 17.0 + z * R;
-    R = 2.0 / 15.0 + z * R;
-    R = 2.0 / 13.0 + z * R;
-    R = 2.0 / 11.0 + z * R;
-    R = 2.0 / 9.0 + z * R;
-    R = 2.0 / 7.0 + z * R;
-    R = 2.0 / 5.0 + z * R;
-    R = 2.0 / 3.0 + z * R;
-    R = z * R;
-    double hfsq = 0.5 * (f * f);
-    double r = f - (hfsq - s * (hfsq + R));
-    double de = (double)e;
-    return de * LN2_HI + (de * LN2_LO + r);
+    int i = (int)((b >> 45) & 127);
+    if (__builtin_fabs(x - 1.0) < 0.0078125) { i = 128; m = x; e = 0; }
+    const double r = __builtin_fma(m, tab[TAB_INVC + i], -1.0);
+    double P = -0.125;
+    P = __builtin_fma(r, P, 1.0 / 7.0);
+    P = __builtin_fma(r, P, -1.0 / 6.0);
+    P = __builtin_fma(r, P, 0.2);
+    P = __builtin_fma(r, P, -0.25);
+    P = __builtin_fma(r, P, 1.0 / 3.0);
+    P = __builtin_fma(r, P, -0.5);
+    const double de = (double)e;
+    const double hi = __builtin_fma(de, LN2_HI, tab[TAB_LHI + i]);
+    const double lo = __builtin_fma(de, LN2_LO, tab[TAB_LLO + i]);
+    const double t = hi + r;
+    const double bb = t - hi;
+    const double err = (hi - (t - bb)) + (r - bb);
+    return t + (err + __builtin_fma(r * r, P, lo));
 }
 
-// expm1(y) for y <= 0: y = k ln2 + r with |r| <= ln2/2, expm1(r) by its Taylor
-// series through r^15/15!, expm1(y) = 2^k p + (2^k - 1).
-__device__ __forceinline__ double dexpm1_neg(double y)
+// expm1(y) for y <= 0: y = k ln2/32 + r with |r| <= ln2/64 (Cody-Waite in two
+// fmas), k = 32q + j, exp(y) = 2^q 2^(j/32) exp(r), expm1(r) by its Taylor
+// series through r^8/8!; expm1(y) = (s - 1) + (s expm1(r) + s_lo) with
+// s + s_lo = 2^q 2^(j/32).
+__device__ __forceinline__ double dexpm1_neg(double y, const double *__restrict__ tab)
 {
     if (y < -45.0) return -1.0;
-    double kd = __builtin_rint(y * 1.4426950408889634);
-    double r = (y - kd * LN2_HI) - kd * LN2_LO;
-    double p = 1.0 / 1307674368000.0;
-    p = 1.0 / 87178291200.0 + r * p;
-    p = 1.0 / 6227020800.0 + r * p;
-    p = 1.0 / 479001600.0 + r * p;
-    p = 1.0 / 39916800.0 + r * p;
-    p = 1.0 / 3628800.0 + r * p;
-    p = 1.0 / 362880.0 + r * p;
-    p = 1.0 / 40320.0 + r * p;
-    p = 1.0 / 5040.0 + r * p;
-    p = 1.0 / 720.0 + r * p;
-    p = 1.0 / 120.0 + r * p;
-    p = 1.0 / 24.0 + r * p;
-    p = 1.0 / 6.0 + r * p;
-    p = 0.5 + r * p;
-    p = r + (r * r) * p;
-    int k = (int)kd;
+    const double kd = __builtin_rint(y * 46.16624130844683);    // ~32/ln2 (only picks k)
+    double r = __builtin_fma(-kd, LN2_HI / 32.0, y);
+    r = __builtin_fma(-kd, LN2_LO / 32.0, r);
+    double P = 1.0 / 40320.0;
+    P = __builtin_fma(r, P, 1.0 / 5040.0);
+    P = __builtin_fma(r, P, 1.0 / 720.0);
+    P = __builtin_fma(r, P, 1.0 / 120.0);
+    P = __builtin_fma(r, P, 1.0 / 24.0);
+    P = __builtin_fma(r, P, 1.0 / 6.0);
+    P = __builtin_fma(r, P, 0.5);
+    const double p = __builtin_fma(r * r, P, r);
+    const int k = (int)kd;
     if (k == 0) return p;
-    double two_k = __longlong_as_double((long long)((uint64_t)(k + 1023) << 52));
-    return two_k * p + (two_k - 1.0);
+    const int j = k & 31, q = k >> 5;
+    const double s = __builtin_ldexp(tab[TAB_EHI + j], q);
+    const double slo = __builtin_ldexp(tab[TAB_ELO + j], q);
+    return (s - 1.0) + __builtin_fma(s, p, slo);
 }
 
 // tanh(x) = -expm1(-2|x|) / (2 + expm1(-2|x|)), odd extension.
-__device__ __forceinline__ double dtanh(double x)
+__device__ __forceinline__ double dtanh(double x, const double *__restrict__ tab)
 {
     if (__builtin_isnan(x)) return x;
     double a = __builtin_fabs(x);
@@ -87,7 +91,7 @@ __device__ __forceinline__ double dtanh(double x)
     if (a < 3.725290298461914e-09) t = a;
     else if (a > 22.0) t = 1.0;
     else {
-        double e = dexpm1_neg(-2.0 * a);
+        double e = dexpm1_neg(-2.0 * a, tab);
         t = -e / (2.0 + e);
     }
     return (x < 0.0) ? -t : t;

@@ -68,6 +68,7 @@ struct Fields {
     double *sinf, *cosf;         // sin(k0 x_i), cos(k0 x_i)
     double *ktab;                // wavenumber table
     double2 *tw;                 // FFT twiddles
+    double *mtab;                // log/expm1 table (devmath.h layout)
     Scalars *sc;
     double *diag;                // 3 diagnostic sums (kinetic, strain, phase)
     EnsSlot *slot = nullptr;     // host-mapped progress record (ensemble boxes only)
@@ -105,7 +106,9 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s);
+void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s);
+// The log/expm1 table of devmath.h, MATH_TAB doubles (host).
+void build_math_table(double *out);
 void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s);
 
 }  // namespace mhit

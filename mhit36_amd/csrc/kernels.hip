@@ -55,19 +55,23 @@ static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box e
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
-// FP64: the project's own log/tanh (bit-identical to the oracle).  FP32 (the
+// FP64: the project's own table-driven log/tanh (bit-identical to the
+// oracle; `tab` is the block's LDS copy of the table).  FP32 (the
 // mixed-precision pass): the device libm.
-__device__ __forceinline__ double tlog(double x) { return dlog(x); }
-__device__ __forceinline__ float tlog(float x) { return logf(x); }
-__device__ __forceinline__ double ttanh(double x) { return dtanh(x); }
-__device__ __forceinline__ float ttanh(float x) { return tanhf(x); }
+__device__ __forceinline__ double tlog(double x, const double *tab) { return dlog(x, tab); }
+__device__ __forceinline__ float tlog(float x, const double *) { return logf(x); }
+__device__ __forceinline__ double ttanh(double x, const double *tab) { return dtanh(x, tab); }
+__device__ __forceinline__ float ttanh(float x, const double *) { return tanhf(x); }
 
 template <typename T>
 __global__ void __launch_bounds__(ATHREADS) k_acdi(
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
-    const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, Consts c, int kz)
+    const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, const double *__restrict__ mtab,
+    Consts c, int kz)
 {
     if (halted(sc)) return;
+    constexpr int NTAB = sizeof(T) == 8 ? MATH_TAB : 1;
+    __shared__ double tabL[NTAB];
     __shared__ T psiL[3][APY * APX];
     __shared__ T sxyL[2][2][ASY * ASX];
     __shared__ T szL[3][ASY * ASX];
@@ -110,7 +114,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
                 T f = pre[q];
                 T val = f < (T)1.0 ? f : (T)1.0;
                 if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
-                psiL[slot][e] = eps * tlog((val + en) / (((T)1.0 - val) + en));
+                psiL[slot][e] = eps * tlog((val + en) / (((T)1.0 - val) + en), tabL);
             }
         }
     };
@@ -133,7 +137,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
                 T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
                 T gz = psiL[psp][pc] - psiL[psm][pc];
                 T normod = (T)1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi);
+                T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi, tabL);
                 T a = (T)0.25 * ((T)1.0 - th * th);
                 sxyL[sxy][0][e] = a * (gx * normod);
                 sxyL[sxy][1][e] = a * (gy * normod);
@@ -159,6 +163,11 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
             nw0 = w[cxy + z];
         }
     };
+
+    if constexpr (sizeof(T) == 8) {
+        for (int q = tid; q < MATH_TAB; q += ATHREADS) tabL[q] = mtab[q];
+        __syncthreads();
+    }
 
     // prologue: psi and phi for planes k0-2 .. k0+1, s for planes k0-1, k0.
     // The psi ring has 3 slots, so s(k0-1) must read psi(k0-2) before
@@ -1135,15 +1144,16 @@ __global__ void __launch_bounds__(64) k_diag_total(const double *__restrict__ ps
 }
 
 // Diagnostic: the device's elementary functions on an array of arguments.
-__global__ void k_debug_math(int op, size_t count, const double *__restrict__ x, double *__restrict__ y)
+__global__ void k_debug_math(int op, size_t count, const double *__restrict__ x, double *__restrict__ y,
+                             const double *__restrict__ mtab)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     double a = x[t];
     double r;
     switch (op) {
-    case 0: r = dlog(a); break;
-    case 1: r = dtanh(a); break;
+    case 0: r = dlog(a, mtab); break;
+    case 1: r = dtanh(a, mtab); break;
     case 2: r = sqrt(a); break;
     default: r = 1.0 / a; break;
     }
@@ -1166,10 +1176,10 @@ void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *ou
     hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(64), 0, s, f.psum, out, c.n);
 }
 
-void launch_debug_math(int op, size_t count, const double *x, double *y, hipStream_t s)
+void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s)
 {
     if (count == 0) return;
-    hipLaunchKernelGGL(k_debug_math, dim3(nblocks(count, BLK)), dim3(BLK), 0, s, op, count, x, y);
+    hipLaunchKernelGGL(k_debug_math, dim3(nblocks(count, BLK)), dim3(BLK), 0, s, op, count, x, y, mtab);
 }
 
 static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
@@ -1191,9 +1201,10 @@ void launch_acdi(const Fields &f, const Consts &c, const Tuning &tu, hipStream_t
     dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, n / kz);
     if (c.mixed)
         hipLaunchKernelGGL(k_acdi<float>, grid, dim3(ATHREADS), 0, s, (const float *)f.u, (const float *)f.v,
-                           (const float *)f.w, (const float *)f.phi, (float *)f.phin, f.sc, c, kz);
+                           (const float *)f.w, (const float *)f.phi, (float *)f.phin, f.sc, f.mtab, c, kz);
     else
-        hipLaunchKernelGGL(k_acdi<double>, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, c, kz);
+        hipLaunchKernelGGL(k_acdi<double>, grid, dim3(ATHREADS), 0, s, f.u, f.v, f.w, f.phi, f.phin, f.sc, f.mtab, c,
+                           kz);
     li->name = "acdi"; li->bytes = (c.mixed ? 2.5 : 5.0) * F(c);
 }
 

@@ -4,6 +4,7 @@
 // reduction partials and a HIP stream.  mhit_step enqueues the kernels of
 // kernels.hip in the order of multi/main.f90:329-946 and synchronises once.
 #include "internal.h"
+#include "devmath.h"
 #include "../../include/mhit.h"
 
 #include <hip/hip_runtime.h>
@@ -305,6 +306,29 @@ extern "C" int mhit_destroy(mhit_ctx *ctx)
     return MHIT_OK;
 }
 
+// The table of the device log/expm1 (devmath.h): 1/c_i to 8 significant
+// bits and -log of it as a double-double (long double logl), 2^(j/32) as a
+// double-double (long double exp2l).
+void mhit::build_math_table(double *t)
+{
+    for (int i = 0; i < 128; i++) {
+        const double c = 1.0 + ((double)i + 0.5) / 128.0;
+        const double invc = rint(256.0 / c) / 256.0;
+        const long double L = -logl((long double)invc);
+        t[TAB_INVC + i] = invc;
+        t[TAB_LHI + i] = (double)L;
+        t[TAB_LLO + i] = (double)(L - (long double)t[TAB_LHI + i]);
+    }
+    t[TAB_INVC + 128] = 1.0;
+    t[TAB_LHI + 128] = 0.0;
+    t[TAB_LLO + 128] = 0.0;
+    for (int j = 0; j < 32; j++) {
+        const long double E = exp2l((long double)j / 32.0L);
+        t[TAB_EHI + j] = (double)E;
+        t[TAB_ELO + j] = (double)(E - (long double)t[TAB_EHI + j]);
+    }
+}
+
 static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipStream_t stream);
 static void cached_tuning(mhit_ctx *c);
 
@@ -399,17 +423,21 @@ static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipSt
     TRY(dalloc(c, (void **)&f.cosf, sizeof(double) * n));
     TRY(dalloc(c, (void **)&f.ktab, sizeof(double) * n));
     TRY(dalloc(c, (void **)&f.tw, sizeof(double2) * tw.size()));
+    TRY(dalloc(c, (void **)&f.mtab, sizeof(double) * MATH_TAB));
     TRY(dalloc(c, (void **)&f.sc, sizeof(Scalars)));
     TRY(dalloc(c, (void **)&f.diag, 4 * sizeof(double)));
 #undef TRY
     Scalars s0;
     memset(&s0, 0, sizeof(s0));
     s0.gumax = 1.0;                 // main.f90:325
+    double mtab[MATH_TAB];
+    build_math_table(mtab);
     hipStream_t st = c->stream;
     if (hipMemcpyAsync(f.sinf, sinf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(f.cosf, cosf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(f.ktab, ktab.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(f.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(f.mtab, mtab, sizeof(mtab), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(f.sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         mhit_destroy(c);
@@ -915,22 +943,28 @@ extern "C" int mhit_debug_math(int op, size_t count, const double *x, double *y)
         (void)hipGetLastError();
         return MHIT_ERR_NODEV;
     }
-    double *dx = nullptr, *dy = nullptr;
+    double *dx = nullptr, *dy = nullptr, *dt = nullptr;
     HIPCHK(hipMalloc(&dx, count * sizeof(double)));
-    if (hipMalloc(&dy, count * sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&dy, count * sizeof(double)) != hipSuccess || hipMalloc(&dt, MATH_TAB * sizeof(double)) != hipSuccess) {
         (void)hipFree(dx);
+        if (dy) (void)hipFree(dy);
         return MHIT_ERR_OOM;
     }
+    double tab[MATH_TAB];
+    build_math_table(tab);
     int rc = MHIT_OK;
-    if (hipMemcpy(dx, x, count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
+    if (hipMemcpy(dx, x, count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess)
+        rc = MHIT_ERR_HIP;
     if (rc == MHIT_OK) {
-        launch_debug_math(op, count, dx, dy, nullptr);
+        launch_debug_math(op, count, dx, dy, dt, nullptr);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(y, dy, count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
             rc = MHIT_ERR_HIP;
     }
     (void)hipFree(dx);
     (void)hipFree(dy);
+    (void)hipFree(dt);
     return rc;
 }
 
@@ -1076,7 +1110,7 @@ static size_t ctx_bytes(const mhit_params *p)
     const size_t FT = n3 * (p->precision == MHIT_MIXED ? 4 : 8);
     const size_t ld = ((n / 2 + 1) + 7) / 8 * 8;
     size_t b = 9 * FT + F + (p->phiflag ? 2 * FT : 0) + 16 * n * n * ld;
-    b += 8 * (6 * n * n + 6 * n + 4 * n) + 65536;
+    b += 8 * (6 * n * n + 6 * n + 4 * n) + 131072;
     return b;
 }
 

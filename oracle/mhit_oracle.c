@@ -40,15 +40,54 @@
 /* Elementary functions shared (as an algorithm) with the device code.       */
 /* ------------------------------------------------------------------------ */
 /*
- * log: x = 2^e * m with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f);
- * log(1+f) = 2 atanh(s) = 2s + s*R(s^2), R(z) = sum_{k>=1} 2/(2k+1) z^k
- * (Taylor series of atanh), written as f - (hfsq - s*(hfsq + R)) with
- * hfsq = f*f/2, which equals 2s + s*R exactly in real arithmetic and keeps f
- * exact.  e*ln2 is split Cody-Waite style: LN2_HI has its 21 low mantissa
- * bits cleared so that e*LN2_HI is exact for |e| < 2^11.
+ * Table-driven log and expm1.  The table is built once, with long double
+ * logl/exp2l for the double-double entries:
+ *   invc[i] = 1/c_i rounded to 8 significant bits, c_i = 1 + (i+1/2)/128
+ *             (i < 128), invc[128] = 1 (the |x-1| < 2^-7 path);
+ *   lhi/llo[i] = -log(invc[i]) as a double-double;
+ *   ehi/elo[j] = 2^(j/32) as a double-double.
+ * log: x = 2^e m, m in [1,2), i = top 7 fraction bits of m, r = m*invc_i - 1
+ * (exact: invc_i has 8 significant bits, |r| <= 2^-7; one fma), log x =
+ * e ln2 - log(invc_i) + log1p(r), log1p by its Taylor series through r^8;
+ * within 2^-7 of 1, r = x - 1 and the table term is 0; the leading sum is
+ * formed error-free (Knuth's TwoSum).  e*ln2 is split Cody-Waite style:
+ * LN2_HI has its 21 low mantissa bits cleared so that e*LN2_HI is exact for
+ * |e| < 2^11.
+ * expm1 (y <= 0): y = k ln2/32 + r, |r| <= ln2/64 (two fmas), k = 32q + j,
+ * expm1(r) by Taylor through r^8/8!, expm1(y) = (s-1) + (s expm1(r) + s_lo)
+ * with s + s_lo = 2^q 2^(j/32).
  */
 static const double LN2_HI = 6.93147180369123816490e-01; /* 0x3FE62E42FEE00000 */
 static const double LN2_LO = 1.90821492927058770002e-10; /* ln2 - LN2_HI       */
+
+static double T_INVC[129], T_LHI[129], T_LLO[129], T_EHI[32], T_ELO[32];
+
+__attribute__((constructor)) static void orc_math_tables(void)
+{
+    for (int i = 0; i < 128; i++) {
+        double c = 1.0 + ((double)i + 0.5) / 128.0;
+        double invc = rint(256.0 / c) / 256.0;
+        long double L = -logl((long double)invc);
+        T_INVC[i] = invc;
+        T_LHI[i] = (double)L;
+        T_LLO[i] = (double)(L - (long double)T_LHI[i]);
+    }
+    T_INVC[128] = 1.0; T_LHI[128] = 0.0; T_LLO[128] = 0.0;
+    for (int j = 0; j < 32; j++) {
+        long double E = exp2l((long double)j / 32.0L);
+        T_EHI[j] = (double)E;
+        T_ELO[j] = (double)(E - (long double)T_EHI[j]);
+    }
+}
+
+void orc_math_table(double *out)            /* the layout the device copy uses */
+{
+    memcpy(out, T_INVC, sizeof(T_INVC));
+    memcpy(out + 129, T_LHI, sizeof(T_LHI));
+    memcpy(out + 258, T_LLO, sizeof(T_LLO));
+    memcpy(out + 387, T_EHI, sizeof(T_EHI));
+    memcpy(out + 419, T_ELO, sizeof(T_ELO));
+}
 
 static inline uint64_t dbits(double x) { uint64_t b; memcpy(&b, &x, 8); return b; }
 static inline double bitsd(uint64_t b) { double x; memcpy(&x, &b, 8); return x; }
@@ -62,57 +101,45 @@ double orc_log(double x)
     uint64_t b = dbits(x);
     e += (int)((b >> 52) & 0x7ff) - 1023;
     double m = bitsd((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);   /* [1,2) */
-    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
-    double f = m - 1.0;
-    double s = f / (2.0 + f);
-    double z = s * s;
-    double R = 2.0 / 25.0;
-    R = 2.0 / 23.0 + z * R;
-    R = 2.0 / 21.0 + z * R;
-    R = 2.0 / 19.0 + z * R;
-    R = 2.0 / 17.0 + z * R;
-    R = 2.0 / 15.0 + z * R;
-    R = 2.0 / 13.0 + z * R;
-    R = 2.0 / 11.0 + z * R;
-    R = 2.0 / 9.0 + z * R;
-    R = 2.0 / 7.0 + z * R;
-    R = 2.0 / 5.0 + z * R;
-    R = 2.0 / 3.0 + z * R;
-    R = z * R;
-    double hfsq = 0.5 * (f * f);
-    double r = f - (hfsq - s * (hfsq + R));
+    int i = (int)((b >> 45) & 127);
+    if (fabs(x - 1.0) < 0.0078125) { i = 128; m = x; e = 0; }
+    double r = fma(m, T_INVC[i], -1.0);
+    double P = -0.125;
+    P = fma(r, P, 1.0 / 7.0);
+    P = fma(r, P, -1.0 / 6.0);
+    P = fma(r, P, 0.2);
+    P = fma(r, P, -0.25);
+    P = fma(r, P, 1.0 / 3.0);
+    P = fma(r, P, -0.5);
     double de = (double)e;
-    return de * LN2_HI + (de * LN2_LO + r);
+    double hi = fma(de, LN2_HI, T_LHI[i]);
+    double lo = fma(de, LN2_LO, T_LLO[i]);
+    double t = hi + r;
+    double bb = t - hi;
+    double err = (hi - (t - bb)) + (r - bb);
+    return t + (err + fma(r * r, P, lo));
 }
 
-/*
- * expm1 on y <= 0 (all tanh needs): y = k*ln2 + r, |r| <= ln2/2, expm1(r) by
- * its Taylor series to r^15/15!, then expm1(y) = 2^k*p + (2^k - 1).
- */
 static double orc_expm1_neg(double y)
 {
     if (y < -45.0) return -1.0;
-    double kd = nearbyint(y * 1.4426950408889634);      /* y / ln2 */
-    double r = (y - kd * LN2_HI) - kd * LN2_LO;
-    double p = 1.0 / 1307674368000.0;                     /* 1/15! */
-    p = 1.0 / 87178291200.0 + r * p;
-    p = 1.0 / 6227020800.0 + r * p;
-    p = 1.0 / 479001600.0 + r * p;
-    p = 1.0 / 39916800.0 + r * p;
-    p = 1.0 / 3628800.0 + r * p;
-    p = 1.0 / 362880.0 + r * p;
-    p = 1.0 / 40320.0 + r * p;
-    p = 1.0 / 5040.0 + r * p;
-    p = 1.0 / 720.0 + r * p;
-    p = 1.0 / 120.0 + r * p;
-    p = 1.0 / 24.0 + r * p;
-    p = 1.0 / 6.0 + r * p;
-    p = 0.5 + r * p;
-    p = r + (r * r) * p;                                  /* expm1(r) */
+    double kd = nearbyint(y * 46.16624130844683);        /* ~32/ln2, only picks k */
+    double r = fma(-kd, LN2_HI / 32.0, y);
+    r = fma(-kd, LN2_LO / 32.0, r);
+    double P = 1.0 / 40320.0;
+    P = fma(r, P, 1.0 / 5040.0);
+    P = fma(r, P, 1.0 / 720.0);
+    P = fma(r, P, 1.0 / 120.0);
+    P = fma(r, P, 1.0 / 24.0);
+    P = fma(r, P, 1.0 / 6.0);
+    P = fma(r, P, 0.5);
+    double p = fma(r * r, P, r);                          /* expm1(r) */
     int k = (int)kd;
     if (k == 0) return p;
-    double two_k = bitsd((uint64_t)(k + 1023) << 52);     /* k in [-65, 0] */
-    return two_k * p + (two_k - 1.0);
+    int j = k & 31, q = k >> 5;                           /* k = 32q + j, arithmetic shift */
+    double s = ldexp(T_EHI[j], q);
+    double slo = ldexp(T_ELO[j], q);
+    return (s - 1.0) + fma(s, p, slo);
 }
 
 /* tanh(x) = -expm1(-2|x|) / (2 + expm1(-2|x|)), odd extension. */

@@ -95,6 +95,7 @@ double orc_dissipation(const double *u, const double *v, const double *w, int n,
 double orc_kinetic_energy(const double *u, const double *v, const double *w, int n);
 double orc_log(double x);
 double orc_tanh(double x);
+void   orc_math_table(double *out);                       /* 451 doubles: log/expm1 table */
 
 #ifdef __cplusplus
 }
