@@ -184,6 +184,15 @@ int mhit_get_tuning(mhit_ctx *ctx, mhit_tuning *out);
  * other value must be one of the knob's valid shapes (MHIT_ERR_ARG if not). */
 int mhit_set_tuning(mhit_ctx *ctx, const mhit_tuning *in);
 
+/* Drop count and sizes of the context's phi, as post/DropCountSize computes
+ * them from a dumped phi (get_interface.f90:11-47, flood_fill.f90,
+ * calculate_deq.f90): cells with phi >= 0.5, 26-connected with periodic wrap;
+ * drops in raster order of their first cell (x fastest, then y, then z);
+ * volume[q] in cells, deq[q] = dx (6 volume/pi)^(1/3).  *n_drops is the total
+ * count; the first max_drops entries are filled.  Needs phiflag = 1 and
+ * 4 N^3 bytes of device memory for the labels (allocated per call). */
+int mhit_drop_count(mhit_ctx *ctx, int max_drops, int *n_drops, double *volume, double *deq);
+
 /* Diagnostics (no counterpart in the reference): evaluate the device's own
  * elementary functions on `count` arguments (op 0 log, 1 tanh, 2 sqrt,
  * 3 reciprocal), and copy an intermediate device buffer of the last step

@@ -28,12 +28,8 @@ constexpr int TAB_INVC = 0, TAB_LHI = 129, TAB_LLO = 258, TAB_EHI = 387, TAB_ELO
 // and one fma forms it; log x = e ln2 - log(invc_i) + log1p(r), log1p by its
 // Taylor series through r^8.  Within 2^-7 of 1, r = x - 1 (exact) and the
 // table term is 0.  The leading sum is formed error-free (Knuth's TwoSum).
-__device__ __forceinline__ double dlog(double x, const double *__restrict__ tab)
+__device__ __forceinline__ double dlog_core(double x, int e, const double *__restrict__ tab)
 {
-    if (!(x > 0.0)) return (x == 0.0) ? -__builtin_inf() : __builtin_nan("");
-    if (__builtin_isinf(x)) return x;
-    int e = 0;
-    if (x < 2.2250738585072014e-308) { x = x * 18014398509481984.0; e = -54; }
     const uint64_t b = (uint64_t)__double_as_longlong(x);
     e += (int)((b >> 52) & 0x7ff) - 1023;
     double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
@@ -54,6 +50,18 @@ __device__ __forceinline__ double dlog(double x, const double *__restrict__ tab)
     const double bb = t - hi;
     const double err = (hi - (t - bb)) + (r - bb);
     return t + (err + __builtin_fma(r * r, P, lo));
+}
+
+__device__ __forceinline__ double dlog(double x, const double *__restrict__ tab)
+{
+    int e = 0;
+    if (__builtin_expect(!(__builtin_isnormal(x) && x > 0.0), 0)) {   // zero, negative, NaN, inf, subnormal
+        if (!(x > 0.0)) return (x == 0.0) ? -__builtin_inf() : __builtin_nan("");
+        if (__builtin_isinf(x)) return x;
+        x = x * 18014398509481984.0;                                  // subnormal: scale by 2^54
+        e = -54;
+    }
+    return dlog_core(x, e, tab);
 }
 
 // expm1(y) for y <= 0: y = k ln2/32 + r with |r| <= ln2/64 (Cody-Waite in two

@@ -107,6 +107,14 @@ void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo 
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s);
+// Drop counting: label the phi >= 0.5 cells (26-connected, periodic) by the
+// smallest linear index of their set; list the roots; count cells per root
+// (roots sorted ascending).
+void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream_t s);
+void launch_ccl_roots(const Consts &c, const unsigned *lab, unsigned *roots, unsigned *nroots, unsigned cap,
+                      hipStream_t s);
+void launch_ccl_volume(const Consts &c, const unsigned *lab, const unsigned *roots, int nr, unsigned long long *vol,
+                       hipStream_t s);
 // The log/expm1 table of devmath.h, MATH_TAB doubles (host).
 void build_math_table(double *out);
 void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s);

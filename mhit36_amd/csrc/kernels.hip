@@ -1161,6 +1161,106 @@ __global__ void k_debug_math(int op, size_t count, const double *__restrict__ x,
 }
 
 // ---------------------------------------------------------------------------
+// Drop counting (post/DropCountSize, get_interface.f90:11-47, flood_fill.f90):
+// cells with phi >= 0.5, 26-connected with periodic wrap, one drop per
+// connected set.  Connected-component labelling by lock-free union-find
+// (labels link a larger index to a smaller one with atomicMin, so every set
+// ends labelled by its smallest linear index -- the cell the reference's
+// raster scan meets first -- whatever the order of the atomics).
+// ---------------------------------------------------------------------------
+constexpr unsigned CCL_NONE = 0xffffffffu;
+
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_ccl_init(const T *__restrict__ phi, unsigned *__restrict__ lab, size_t n3)
+{
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK)
+        lab[x] = (double)phi[x] >= 0.5 ? (unsigned)x : CCL_NONE;
+}
+
+__device__ __forceinline__ unsigned ccl_find(const unsigned *lab, unsigned x)
+{
+    const volatile unsigned *l = lab;
+    unsigned p = l[x];
+    while (p != x) {
+        x = p;
+        p = l[x];
+    }
+    return x;
+}
+
+__device__ __forceinline__ void ccl_union(unsigned *lab, unsigned a, unsigned b)
+{
+    a = ccl_find(lab, a);
+    b = ccl_find(lab, b);
+    while (a != b) {
+        if (a < b) { unsigned t = a; a = b; b = t; }
+        const unsigned old = atomicMin(&lab[a], b);     // link the larger root to the smaller
+        if (old == a) return;
+        a = ccl_find(lab, old);                        // a was linked meanwhile: join its root
+        b = ccl_find(lab, b);
+    }
+}
+
+// The 13 "forward" offsets of the 26-neighbourhood (dz > 0, or dz = 0 and
+// dy > 0, or dz = dy = 0 and dx > 0); each pair of cells is visited once.
+__global__ void __launch_bounds__(BLK) k_ccl_merge(unsigned *__restrict__ lab, int n)
+{
+    const size_t n3 = (size_t)n * n * n;
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK) {
+        if (lab[x] == CCL_NONE) continue;
+        const int i = (int)(x % n), j = (int)((x / n) % n), k = (int)(x / ((size_t)n * n));
+        for (int o = 0; o < 13; o++) {
+            int dz = o < 9 ? 1 : 0;
+            int r = o < 9 ? o : o - 9;                  // dz = 1: 9 offsets (dx, dy in -1..1)
+            int dy, dx;
+            if (dz) { dy = r / 3 - 1; dx = r % 3 - 1; }
+            else if (r < 3) { dy = 1; dx = r - 1; }       // dz = 0, dy = 1: 3 offsets
+            else { dy = 0; dx = 1; }                      // dz = dy = 0, dx = 1
+            const int ii = i + dx < 0 ? n - 1 : (i + dx >= n ? 0 : i + dx);
+            const int jj = j + dy < 0 ? n - 1 : (j + dy >= n ? 0 : j + dy);
+            const int kk = k + dz >= n ? 0 : k + dz;
+            const size_t y = (size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk);
+            if (lab[y] != CCL_NONE) ccl_union(lab, (unsigned)x, (unsigned)y);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_ccl_flatten(unsigned *__restrict__ lab, size_t n3)
+{
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK)
+        if (lab[x] != CCL_NONE) lab[x] = ccl_find(lab, (unsigned)x);
+}
+
+__global__ void __launch_bounds__(BLK) k_ccl_roots(const unsigned *__restrict__ lab, size_t n3,
+                                                   unsigned *__restrict__ roots, unsigned *__restrict__ nroots,
+                                                   unsigned cap)
+{
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK)
+        if (lab[x] == (unsigned)x) {
+            const unsigned q = atomicAdd(nroots, 1u);
+            if (q < cap) roots[q] = (unsigned)x;
+        }
+}
+
+// Cells per drop: position of the cell's label among the sorted roots.
+__global__ void __launch_bounds__(BLK) k_ccl_volume(const unsigned *__restrict__ lab, size_t n3,
+                                                    const unsigned *__restrict__ roots, int nr,
+                                                    unsigned long long *__restrict__ vol)
+{
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK) {
+        const unsigned l = lab[x];
+        if (l == CCL_NONE) continue;
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (roots[mid] < l) lo = mid + 1;
+            else hi = mid;
+        }
+        atomicAdd(&vol[lo], 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s)
@@ -1174,6 +1274,37 @@ void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *ou
                            f.lsum, f.sc, c, dxi);
     hipLaunchKernelGGL(k_diag_planes, dim3(c.n, 3), dim3(64), 0, s, f.lsum, f.psum, c.n);
     hipLaunchKernelGGL(k_diag_total, dim3(1), dim3(64), 0, s, f.psum, out, c.n);
+}
+
+// Grid of a grid-stride pass over n3 cells: enough blocks to fill the chip.
+static unsigned ccl_grid(size_t n3)
+{
+    const size_t b = nblocks(n3, BLK);
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+
+void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream_t s)
+{
+    const size_t n3 = (size_t)c.n * c.n * c.n;
+    const unsigned g = ccl_grid(n3);
+    if (c.mixed) hipLaunchKernelGGL(k_ccl_init<float>, dim3(g), dim3(BLK), 0, s, (const float *)f.phi, lab, n3);
+    else hipLaunchKernelGGL(k_ccl_init<double>, dim3(g), dim3(BLK), 0, s, f.phi, lab, n3);
+    hipLaunchKernelGGL(k_ccl_merge, dim3(g), dim3(BLK), 0, s, lab, c.n);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(g), dim3(BLK), 0, s, lab, n3);
+}
+
+void launch_ccl_roots(const Consts &c, const unsigned *lab, unsigned *roots, unsigned *nroots, unsigned cap,
+                      hipStream_t s)
+{
+    const size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_ccl_roots, dim3(ccl_grid(n3)), dim3(BLK), 0, s, lab, n3, roots, nroots, cap);
+}
+
+void launch_ccl_volume(const Consts &c, const unsigned *lab, const unsigned *roots, int nr, unsigned long long *vol,
+                       hipStream_t s)
+{
+    const size_t n3 = (size_t)c.n * c.n * c.n;
+    hipLaunchKernelGGL(k_ccl_volume, dim3(ccl_grid(n3)), dim3(BLK), 0, s, lab, n3, roots, nr, vol);
 }
 
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s)

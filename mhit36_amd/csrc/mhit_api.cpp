@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -932,6 +933,70 @@ extern "C" int mhit_diagnostics(mhit_ctx *c, double dxi, double nu, mhit_stats *
     out->phi_sum = c->kc.phiflag ? sums[2] : 0.0;
     out->phi_volume = out->phi_sum * c->cs.dx * c->cs.dx * c->cs.dx;
     return MHIT_OK;
+}
+
+// post/DropCountSize (get_interface.f90:11-47, flood_fill.f90:1-236,
+// calculate_deq.f90:1-40) on the context's phi.
+extern "C" int mhit_drop_count(mhit_ctx *c, int max_drops, int *n_drops, double *volume, double *deq)
+{
+    if (!c || !n_drops || max_drops < 0 || (max_drops > 0 && (!volume || !deq))) return MHIT_ERR_ARG;
+    if (!c->kc.phiflag) return MHIT_ERR_ARG;
+    const int n = c->kc.n;
+    const size_t n3 = (size_t)n * n * n;
+    if (n3 >= (size_t)0xffffffffu) return MHIT_ERR_BAD_N;          // 32-bit labels
+    const unsigned cap = (unsigned)std::min<size_t>(n3 / 8 + 1, (size_t)1 << 24);
+    unsigned *lab = nullptr, *roots = nullptr, *nroots = nullptr;
+    unsigned long long *vol = nullptr;
+    int rc = MHIT_OK;
+    if (hipMalloc((void **)&lab, n3 * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&roots, (size_t)cap * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&nroots, sizeof(unsigned)) != hipSuccess)
+        rc = MHIT_ERR_OOM;
+    hipStream_t st = c->stream;
+    unsigned nr = 0;
+    std::vector<unsigned> hroots;
+    if (rc == MHIT_OK) {
+        launch_ccl_label(c->f, c->kc, lab, st);
+        if (hipMemsetAsync(nroots, 0, sizeof(unsigned), st) != hipSuccess) rc = MHIT_ERR_HIP;
+        launch_ccl_roots(c->kc, lab, roots, nroots, cap, st);
+        if (rc == MHIT_OK && (hipGetLastError() != hipSuccess ||
+                              hipMemcpyAsync(&nr, nroots, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                              hipStreamSynchronize(st) != hipSuccess))
+            rc = MHIT_ERR_HIP;
+    }
+    if (rc == MHIT_OK && nr > cap) rc = MHIT_ERR_UNSUPPORTED;           // more drops than the root list holds
+    if (rc == MHIT_OK && nr > 0) {
+        hroots.resize(nr);
+        if (hipMemcpyAsync(hroots.data(), roots, nr * sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = MHIT_ERR_HIP;
+        std::sort(hroots.begin(), hroots.end());           // raster order of each drop's first cell
+        if (rc == MHIT_OK && (hipMalloc((void **)&vol, nr * sizeof(unsigned long long)) != hipSuccess)) rc = MHIT_ERR_OOM;
+        std::vector<unsigned long long> hv(nr);
+        if (rc == MHIT_OK &&
+            (hipMemcpyAsync(roots, hroots.data(), nr * sizeof(unsigned), hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipMemsetAsync(vol, 0, nr * sizeof(unsigned long long), st) != hipSuccess))
+            rc = MHIT_ERR_HIP;
+        if (rc == MHIT_OK) {
+            launch_ccl_volume(c->kc, lab, roots, (int)nr, vol, st);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(hv.data(), vol, nr * sizeof(unsigned long long), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                rc = MHIT_ERR_HIP;
+        }
+        // deq = dx (6/pi vol)^(1/3), pi = 4 atan(1), dx = lx/(nx-1) (calculate_deq.f90:31; module.f90:3-5)
+        const double n6opi = 6.0 / (4.0 * atan(1.0));
+        for (int q = 0; rc == MHIT_OK && q < (int)nr && q < max_drops; q++) {
+            volume[q] = (double)hv[q];
+            deq[q] = c->cs.dx * pow(n6opi * volume[q], 1.0 / 3.0);
+        }
+    }
+    if (rc == MHIT_OK) *n_drops = (int)nr;
+    if (lab) (void)hipFree(lab);
+    if (roots) (void)hipFree(roots);
+    if (nroots) (void)hipFree(nroots);
+    if (vol) (void)hipFree(vol);
+    return rc;
 }
 
 extern "C" int mhit_debug_math(int op, size_t count, const double *x, double *y)

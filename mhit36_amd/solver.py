@@ -145,6 +145,19 @@ class Solver:
         check(self._lib.mhit_diagnostics(self._h, float(dxi), float(nu), ctypes.byref(st)), "diagnostics")
         return {name: getattr(st, name) for name, _ in Stats._fields_}
 
+    def drop_count(self, max_drops=1 << 16):
+        """post/DropCountSize on the current phi: number of drops, and their
+        volumes (cells) and equivalent diameters in raster order of each
+        drop's first cell."""
+        cnt = ctypes.c_int(0)
+        vol = np.zeros(max_drops)
+        deq = np.zeros(max_drops)
+        check(self._lib.mhit_drop_count(self._h, int(max_drops), ctypes.byref(cnt),
+                                        vol.ctypes.data_as(ctypes.c_void_p), deq.ctypes.data_as(ctypes.c_void_p)),
+              "drop_count")
+        k = min(cnt.value, max_drops)
+        return cnt.value, vol[:k].copy(), deq[:k].copy()
+
     # -- launch shapes -----------------------------------------------------------
     def autotune(self, force=False):
         """Time the candidate launch shapes of the tunable passes on this fresh

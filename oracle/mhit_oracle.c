@@ -1037,3 +1037,51 @@ double orc_kinetic_energy(const double *u, const double *v, const double *w, int
     double dn = (double)n;
     return 0.5 * (s / dn / dn / dn);
 }
+
+/* ------------------------------------------------------------------------ */
+/* post/DropCountSize (get_interface.f90:11-47, flood_fill.f90:1-236,        */
+/* calculate_deq.f90:24-31; module.f90:3-5): binarize phi >= 0.5, then for   */
+/* each unvisited marked cell in raster order (i fastest, then j, then k)    */
+/* flood-fill its 26-connected set with periodic wrap; volume = cells,       */
+/* deq = dx (6/pi vol)^(1/3), dx = 2 pi/(n-1).  The recursion of the         */
+/* reference is replaced by an explicit stack (same set, same order of       */
+/* drops).  Returns the number of drops; fills the first `max` entries.      */
+/* ------------------------------------------------------------------------ */
+int orc_drop_count(const double *phi, int n, int max, double *vol, double *deq)
+{
+    const size_t n3 = (size_t)n * n * n;
+    unsigned char *top = (unsigned char *)malloc(n3);
+    size_t *stack = (size_t *)malloc(n3 * sizeof(size_t));
+    if (!top || !stack) { free(top); free(stack); return -1; }
+    for (size_t x = 0; x < n3; x++) top[x] = phi[x] >= 0.5 ? 1 : 0;
+    const double pi = 4.0 * atan(1.0), n6opi = 6.0 / pi;
+    const double dx = (8.0 * atan(1.0)) / (double)(n - 1);
+    int count = 0;
+    for (size_t x0 = 0; x0 < n3; x0++) {
+        if (!top[x0]) continue;
+        size_t sp = 0, cells = 0;
+        stack[sp++] = x0;
+        top[x0] = 0;
+        while (sp) {
+            size_t x = stack[--sp];
+            cells++;
+            int i = (int)(x % n), j = (int)((x / n) % n), k = (int)(x / ((size_t)n * n));
+            for (int dk = -1; dk <= 1; dk++)
+                for (int dj = -1; dj <= 1; dj++)
+                    for (int di = -1; di <= 1; di++) {
+                        if (!di && !dj && !dk) continue;
+                        int ii = (i + di + n) % n, jj = (j + dj + n) % n, kk = (k + dk + n) % n;
+                        size_t y = (size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk);
+                        if (top[y]) { top[y] = 0; stack[sp++] = y; }
+                    }
+        }
+        if (count < max) {
+            vol[count] = (double)cells;
+            deq[count] = dx * pow(n6opi * vol[count], 1.0 / 3.0);
+        }
+        count++;
+    }
+    free(top);
+    free(stack);
+    return count;
+}

@@ -96,6 +96,7 @@ double orc_kinetic_energy(const double *u, const double *v, const double *w, int
 double orc_log(double x);
 double orc_tanh(double x);
 void   orc_math_table(double *out);                       /* 451 doubles: log/expm1 table */
+int    orc_drop_count(const double *phi, int n, int max, double *vol, double *deq);   /* DropCountSize */
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,8 @@ def load():
         "orc_kinetic_energy": ([dp, dp, dp, ctypes.c_int], ctypes.c_double),
         "orc_log": ([ctypes.c_double], ctypes.c_double),
         "orc_tanh": ([ctypes.c_double], ctypes.c_double),
+        "orc_math_table": ([dp], None),
+        "orc_drop_count": ([dp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -214,6 +216,25 @@ def log(x):
 
 def tanh(x):
     return load().orc_tanh(float(x))
+
+
+def drop_count(phi, max_drops=1 << 16):
+    """post/DropCountSize restated: (count, volumes in cells, equivalent
+    diameters) of the phi >= 0.5 sets, 26-connected, periodic, in raster order."""
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    vol = np.zeros(max_drops)
+    deq = np.zeros(max_drops)
+    cnt = load().orc_drop_count(_dp(phi), phi.shape[0], int(max_drops), _dp(vol), _dp(deq))
+    if cnt < 0:
+        raise MemoryError("orc_drop_count")
+    k = min(cnt, max_drops)
+    return cnt, vol[:k].copy(), deq[:k].copy()
+
+
+def math_table():
+    t = np.zeros(451)
+    load().orc_math_table(_dp(t))
+    return t
 
 
 def set_threads(n):
