@@ -99,8 +99,24 @@ contains
     call mhit_check(mhit_write_field(ctx, cstr('./output'), tt, 1), 'writefield u')
     call mhit_check(mhit_write_field(ctx, cstr('./output'), tt, 2), 'writefield v')
     call mhit_check(mhit_write_field(ctx, cstr('./output'), tt, 3), 'writefield w')
-    if (p%phiflag == 1) call mhit_check(mhit_write_field(ctx, cstr('./output'), tt, 5), 'writefield phi')
+    if (p%phiflag == 1) then
+       call mhit_check(mhit_write_field(ctx, cstr('./output'), tt, 5), 'writefield phi')
+       call count_drops(tt)
+    end if
   end subroutine dump_fields
+
+  ! post/DropCountSize on the device: the drop count of this step, appended to
+  ! ./output/drop_count.dat as that tool writes it (get_interface.f90:40-45).
+  subroutine count_drops(tt)
+    integer(c_int), intent(in) :: tt
+    integer(c_int) :: nd
+    real(c_double) :: vol(1), deq(1)
+    call mhit_check(mhit_drop_count(ctx, 0_c_int, nd, vol, deq), 'drop_count')
+    write(*,'(2x,a,i4)') 'Number of drops: ', nd
+    open(2, file='./output/drop_count.dat', access='append', form='formatted', status='unknown')
+    write(2,'(i16,2x,i16)') tt, nd
+    close(2, status='keep')
+  end subroutine count_drops
 
   subroutine read_restart(tt, fieldn, name)
     integer(c_int), intent(in) :: tt, fieldn

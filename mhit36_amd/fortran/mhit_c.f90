@@ -90,6 +90,13 @@ module mhit_c
        character(kind=c_char), dimension(*), intent(in) :: path
        integer(c_int), value :: fieldn
      end function
+     integer(c_int) function mhit_drop_count(ctx, max_drops, n_drops, volume, deq) bind(C, name="mhit_drop_count")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: max_drops
+       integer(c_int), intent(out) :: n_drops
+       real(c_double), dimension(*), intent(out) :: volume, deq
+     end function
      type(c_ptr) function mhit_strerror_c(code) bind(C, name="mhit_strerror")
        import :: c_int, c_ptr
        integer(c_int), value :: code

@@ -129,3 +129,9 @@ def test_fortran_driver_end_to_end(tmp_path):
         for f, name in ((1, "u"), (2, "v"), (3, "w"), (5, "phi")):
             raw = np.fromfile(tmp_path / "output" / f"{name}_{t:08d}.dat", dtype="<f8").reshape((n, n, n), order="F")
             assert np.array_equal(raw, o.get_field(f)), (t, name)
+    # drop count at every dump, as post/DropCountSize writes it
+    counts = [tuple(int(x) for x in l.split()) for l in open(tmp_path / "output" / "drop_count.dat")]
+    assert [t for t, _ in counts] == [0, 3, 6]
+    for t, nd in counts:
+        phi = np.fromfile(tmp_path / "output" / f"phi_{t:08d}.dat", dtype="<f8").reshape((n, n, n), order="F")
+        assert nd == orc.drop_count(phi)[0] == 1
