@@ -56,12 +56,21 @@ static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box e
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
 // FP64: the project's own table-driven log/tanh (bit-identical to the
-// oracle; `tab` is the block's LDS copy of the table).  FP32 (the
-// mixed-precision pass): the device libm.
+// oracle; `tab` is the block's LDS copy of the table) and IEEE division.
+// FP32 (the mixed-precision pass, compared with FP64 by tolerance): the
+// hardware log2/exp2 approximations and a reciprocal-based quotient (a few
+// ulp of FP32, far below the FP32 storage error of the state).
 __device__ __forceinline__ double tlog(double x, const double *tab) { return dlog(x, tab); }
-__device__ __forceinline__ float tlog(float x, const double *) { return logf(x); }
+__device__ __forceinline__ float tlog(float x, const double *) { return __logf(x); }
 __device__ __forceinline__ double ttanh(double x, const double *tab) { return dtanh(x, tab); }
-__device__ __forceinline__ float ttanh(float x, const double *) { return tanhf(x); }
+__device__ __forceinline__ float ttanh(float x, const double *)
+{
+    const float e = __expf(-2.0f * __builtin_fabsf(x));          // tanh|x| = (1 - e)/(1 + e)
+    const float t = __fdividef(1.0f - e, 1.0f + e);
+    return x < 0.0f ? -t : t;
+}
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float tdiv(float a, float b) { return __fdividef(a, b); }
 
 template <typename T>
 __global__ void __launch_bounds__(ATHREADS) k_acdi(
@@ -114,7 +123,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
                 T f = pre[q];
                 T val = f < (T)1.0 ? f : (T)1.0;
                 if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
-                psiL[slot][e] = eps * tlog((val + en) / (((T)1.0 - val) + en), tabL);
+                psiL[slot][e] = eps * tlog(tdiv(val + en, ((T)1.0 - val) + en), tabL);
             }
         }
     };
@@ -136,7 +145,7 @@ __global__ void __launch_bounds__(ATHREADS) k_acdi(
                 T gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
                 T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
                 T gz = psiL[psp][pc] - psiL[psm][pc];
-                T normod = (T)1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+                T normod = tdiv((T)1.0, (T)(sqrt((gx * gx + gy * gy) + gz * gz) + tiny));
                 T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi, tabL);
                 T a = (T)0.25 * ((T)1.0 - th * th);
                 sxyL[sxy][0][e] = a * (gx * normod);
