@@ -472,7 +472,8 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
 // wavefront are contiguous for column tiles.
 // ---------------------------------------------------------------------------
 constexpr int FFT_MAX_THREADS = 1024;
-constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block
+constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block of the heuristic shapes (2 blocks per CU)
+constexpr size_t LDS_MAX = 163840;  // what a gfx950 workgroup may allocate (tunable shapes)
 
 template <int R, int SG>
 __device__ __forceinline__ void fft_stage(double2 *lds, int n, int nlines, int lstride, int pstride,
@@ -1444,7 +1445,7 @@ static int fft_threads(const FftPlan &pl, int nlines)
 
 // Tunable launch shapes.  Every candidate satisfies what the kernels assume:
 // z planes per block divide n; blocks have at most FFT_MAX_THREADS threads and
-// LDS_CAP bytes of LDS; pairs per block divide the n^2/2 line pairs.
+// LDS_MAX bytes of LDS; pairs per block divide the n^2/2 line pairs.
 int knob_default(int knob, const Consts &c, const FftPlan &pl)
 {
     const bool regs = plan_regs(pl);
@@ -1469,10 +1470,10 @@ int knob_candidates(int knob, const Consts &c, const FftPlan &pl, int *out, int 
             ok = v >= 2 && v <= n && n % v == 0;
         } else if (knob == KNOB_XPAIRS || knob == KNOB_XIPAIRS) {
             const size_t lds = sizeof(double2) * (size_t)v * (regs ? rline_stride(n) : n + 1);
-            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_CAP && (n * n / 2) % v == 0;
+            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_MAX && (n * n / 2) % v == 0;
         } else {
             const size_t lds = sizeof(double2) * (size_t)v * n;
-            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_CAP && v <= n / 2 + 1;
+            ok = v <= 16 && v * per_line <= FFT_MAX_THREADS && lds <= LDS_MAX && v <= n / 2 + 1;
         }
         if (ok) out[k++] = v;
     }
