@@ -89,6 +89,25 @@ def cpu_baseline(n, phiflag, fields, dt, steps=2):
                       f"{steps} steps, the last one timed, {threads} OpenMP threads"}
 
 
+def cpu_baseline_1thread(n=256, phiflag=1):
+    """The oracle on one host thread, on the same generators at 256^3 (one
+    512^3 step would take ~40 s on one thread): one untimed step, one timed."""
+    from oracle import orc
+    u, v, w, phi, dt = make_inputs(n, phiflag, ndrops=64, radius=0.2)
+    orc.set_threads(1)
+    o = orc.Oracle(**params_for(n, phiflag, dt))
+    for f, a in {1: u, 2: v, 3: w, 5: phi}.items():
+        o.set_field(f, a)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(1)
+    ms = (time.perf_counter() - t0) * 1e3
+    o.close()
+    return {"value": round(ms, 3), "unit": "ms/timestep", "cores": 1, "kind": "port",
+            "sample": f"oracle/liboracle.so, {n}^3 NS+ACDI (same generators, 64 drops R=0.2), "
+                      "2 steps, the last one timed, 1 thread"}
+
+
 def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
     """Measured stream-copy ceiling: a plain 1 GiB device-to-device copy
     (read + write bytes over its event-timed duration)."""
@@ -233,9 +252,10 @@ def main():
         print(f"[bench] ensemble {ens['ensemble_s']:.3f}s vs one at a time {ens['one_at_a_time_s']:.3f}s, "
               f"p99 {ens['step_latency_p99_ms']:.3f} ms", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, phiflag, fields, dt)
+        cpu1 = cpu_baseline_1thread()
 
     out = {
         "metric": "ms/timestep (FP64 vs mixed) at 512^3 NS+ACDI; HBM GB/s; p99 ensemble latency",
@@ -261,6 +281,7 @@ def main():
         "stream_copy_gbs": copy_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "mixed": mixed,
         "tuning": {k: v["value"] for k, v in tunings.items()},
         "ensemble": ens,
