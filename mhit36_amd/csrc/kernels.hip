@@ -464,6 +464,267 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
 }
 
 // ---------------------------------------------------------------------------
+// The same pass for FP32 state with two x-points per thread: 64 x 8 points
+// per block, every global load and store a float2 (the u/v/w and phi boxes
+// start at the even column i0-2, so pairs never straddle the periodic seam),
+// half the memory instructions per byte of the one-point form.  Each point's
+// arithmetic is mom_point below, written exactly as in k_momentum, so the
+// results are those of k_momentum<PHI, float> bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int PTX = 32, PTY = 8, PTHREADS = PTX * PTY;   // threads; 2 points each in x
+constexpr int PBX = 2 * PTX + 4, PUY = PTY + 2;           // u/v/w box row (68) and rows (10)
+constexpr int PPY = PTY + 3;                              // phi box rows (11), same row as u/v/w
+constexpr int PFX = 2 * PTX + 1, PFY = PTY + 1;           // force box (65 x 9)
+static_assert((PBX / 2) * PUY <= 2 * PTHREADS && (PBX / 2) * PPY <= 2 * PTHREADS, "two pairs per thread");
+static_assert(PFX * PFY <= 3 * PTHREADS, "three force points per thread");
+
+struct MomK {
+    float dxi, ddxi, mu, rhoi, f1, f2, f3, dt, alpha, beta;
+};
+
+// A11-A15 at one point, main.f90:513-696 (as in k_momentum).  U0..W0: plane
+// k, Up..Wp: plane k+1, with row stride SX; uzm..wzm, uxpm (u at k-1, i+1)
+// and vypm (v at k-1, j+1) from plane k-1; FX/FY/FZ the force of plane k and
+// fzm its z part at plane k-1 (row stride FXS).
+template <int PHI, int SX, int FXS>
+__device__ __forceinline__ void mom_point(const float *U0, const float *V0, const float *W0, const float *Up,
+                                          const float *Vp, const float *Wp, int uc, float uzm, float vzm, float wzm,
+                                          float uxpm, float vypm, const float *FX, const float *FY, const float *FZ,
+                                          float fzm, int fc, float sk, float cj, float si, float ck, float sj, float ci,
+                                          float ro0, float ro1, float ro2, const MomK &q, float *out)
+{
+    using T = float;
+    const T dxi = q.dxi, ddxi = q.ddxi, mu = q.mu, rhoi = q.rhoi, f1 = q.f1, f2 = q.f2, f3 = q.f3;
+    const T u0 = U0[uc], v0 = V0[uc], w0 = W0[uc];
+    const T uxp = U0[uc + 1], uxm = U0[uc - 1], uyp = U0[uc + SX], uym = U0[uc - SX];
+    const T uzp = Up[uc];
+    const T vxp = V0[uc + 1], vxm = V0[uc - 1], vyp = V0[uc + SX], vym = V0[uc - SX];
+    const T vzp = Vp[uc];
+    const T wxp = W0[uc + 1], wxm = W0[uc - 1], wyp = W0[uc + SX], wym = W0[uc - SX];
+    const T wzp = Wp[uc];
+    T h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+    T h12 = (uyp + u0) * (vyp + V0[uc + SX - 1]) - (u0 + uym) * (v0 + vxm);
+    T h13 = (uzp + u0) * (wzp + Wp[uc - 1]) - (u0 + uzm) * (w0 + wxm);
+    T h21 = (uxp + U0[uc + 1 - SX]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+    T h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+    T h23 = (wzp + Wp[uc - SX]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+    T h31 = (wxp + w0) * (uxp + uxpm) - (w0 + wxm) * (u0 + uzm);
+    T h32 = (vyp + vypm) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+    T h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+    h11 = h11 * (T)0.25 * dxi; h12 = h12 * (T)0.25 * dxi; h13 = h13 * (T)0.25 * dxi;
+    h21 = h21 * (T)0.25 * dxi; h22 = h22 * (T)0.25 * dxi; h23 = h23 * (T)0.25 * dxi;
+    h31 = h31 * (T)0.25 * dxi; h32 = h32 * (T)0.25 * dxi; h33 = h33 * (T)0.25 * dxi;
+    T ru = -((h11 + h12) + h13);
+    T rv = -((h21 + h22) + h23);
+    T rw = -((h31 + h32) + h33);
+    h11 = mu * ((uxp - (T)2.0 * u0) + uxm) * ddxi;
+    h12 = mu * ((uyp - (T)2.0 * u0) + uym) * ddxi;
+    h13 = mu * ((uzp - (T)2.0 * u0) + uzm) * ddxi;
+    h21 = mu * ((vxp - (T)2.0 * v0) + vxm) * ddxi;
+    h22 = mu * ((vyp - (T)2.0 * v0) + vym) * ddxi;
+    h23 = mu * ((vzp - (T)2.0 * v0) + vzm) * ddxi;
+    h31 = mu * ((wxp - (T)2.0 * w0) + wxm) * ddxi;
+    h32 = mu * ((wyp - (T)2.0 * w0) + wym) * ddxi;
+    h33 = mu * ((wzp - (T)2.0 * w0) + wzm) * ddxi;
+    ru = ru + ((h11 + h12) + h13) * rhoi;
+    rv = rv + ((h21 + h22) + h23) * rhoi;
+    rw = rw + ((h31 + h32) + h33) * rhoi;
+    ru = (ru + f3 * sk) + f2 * cj;
+    rv = (rv + f1 * si) + f3 * ck;
+    rw = (rw + f2 * sj) + f1 * ci;
+    if constexpr (PHI) {
+        ru = ru + (T)0.5 * (FX[fc - 1] + FX[fc]) * rhoi;
+        rv = rv + (T)0.5 * (FY[fc - FXS] + FY[fc]) * rhoi;
+        rw = rw + (T)0.5 * (fzm + FZ[fc]) * rhoi;
+    }
+    const T dt = q.dt, alpha = q.alpha, beta = q.beta;
+    out[0] = u0 + dt * (alpha * ru - beta * ro0);
+    out[1] = v0 + dt * (alpha * rv - beta * ro1);
+    out[2] = w0 + dt * (alpha * rw - beta * ro2);
+    out[3] = ru;
+    out[4] = rv;
+    out[5] = rw;
+}
+
+__device__ __forceinline__ float2 ld2(const float *p) { return *reinterpret_cast<const float2 *>(p); }
+__device__ __forceinline__ void st2nt(float *p, float a, float b)
+{
+    typedef float nt_float2 __attribute__((ext_vector_type(2)));
+    nt_float2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_float2 *>(p));
+}
+
+template <int PHI>
+__global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
+    const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
+    const float *__restrict__ phi, float *__restrict__ us, float *__restrict__ vs, float *__restrict__ ws,
+    float *__restrict__ ruo, float *__restrict__ rvo, float *__restrict__ rwo,
+    const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
+    Consts c, double ALPHA, double BETA, int kz)
+{
+    using T = float;
+    if (halted(sc)) return;
+    __shared__ T UL[3][3][PUY * PBX];
+    __shared__ T PL[PHI ? 3 : 1][PHI ? PPY * PBX : 1];
+    __shared__ T FL[PHI ? 2 : 1][3][PHI ? PFY * PFX : 1];
+    const int n = c.n;
+    const int tid = threadIdx.x;
+    const uint3 blk = xcd_block();
+    const int i0 = blk.x * (2 * PTX), j0 = blk.y * PTY, k0 = blk.z * kz;
+    const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
+    MomK q;
+    q.dxi = (T)c.dxi; q.ddxi = (T)c.ddxi; q.mu = (T)c.mu; q.rhoi = (T)c.rhoi;
+    q.f1 = (T)c.f1; q.f2 = (T)c.f2; q.f3 = (T)c.f3;
+    q.dt = (T)c.dt; q.alpha = (T)ALPHA; q.beta = (T)BETA;
+    constexpr int NP = PBX / 2;                    // pairs per box row (34)
+
+    // pairs of this thread in the u/v/w box (rows j0-1 ..) and the phi box (rows j0-2 ..)
+    unsigned uoff[2], foff[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int e = tid + r * PTHREADS;
+        const int x = wrap(i0 - 2 + 2 * (e % NP), n);
+        uoff[r] = (unsigned)x + (unsigned)n * wrap(j0 - 1 + e / NP, n);
+        foff[r] = (unsigned)x + (unsigned)n * wrap(j0 - 2 + e / NP, n);
+    }
+    const size_t nn = (size_t)n * n;
+    float2 pu[2][3], pp[2];
+    auto fetch_u = [&](int m) {
+        const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (tid + r * PTHREADS < NP * PUY) {
+                const size_t g = uoff[r] + zoff;
+                pu[r][0] = ld2(u + g); pu[r][1] = ld2(v + g); pu[r][2] = ld2(w + g);
+            }
+    };
+    auto store_u = [&](int m) {
+        const int slot = (m - k0 + 1) % 3;
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int e = tid + r * PTHREADS;
+            if (e < NP * PUY) {
+                const int b = (e / NP) * PBX + 2 * (e % NP);
+                UL[slot][0][b] = pu[r][0].x - um; UL[slot][0][b + 1] = pu[r][0].y - um;
+                UL[slot][1][b] = pu[r][1].x - vm; UL[slot][1][b + 1] = pu[r][1].y - vm;
+                UL[slot][2][b] = pu[r][2].x - wm; UL[slot][2][b + 1] = pu[r][2].y - wm;
+            }
+        }
+    };
+    auto fetch_phi = [&](int m) {
+        if constexpr (PHI) {
+            const size_t zoff = nn * (size_t)wrap(m, n);
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+                if (tid + r * PTHREADS < NP * PPY) pp[r] = ld2(phi + foff[r] + zoff);
+        }
+    };
+    auto store_phi = [&](int m) {
+        if constexpr (PHI) {
+            const int slot = (m - k0 + 2) % 3;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int e = tid + r * PTHREADS;
+                if (e < NP * PPY) {
+                    const int b = (e / NP) * PBX + 2 * (e % NP);
+                    PL[slot][b] = pp[r].x;
+                    PL[slot][b + 1] = pp[r].y;
+                }
+            }
+        }
+    };
+    auto load_u = [&](int m) { fetch_u(m); store_u(m); };
+    auto load_phi = [&](int m) { fetch_phi(m); store_phi(m); };
+    auto force = [&](int m) {           // A14 first half on plane m (as in k_momentum)
+        if constexpr (PHI) {
+            const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
+            const int fs = (m - k0 + 1) & 1;
+            const T sig6 = (T)((T)6.0 * c.sigma), eps = (T)c.eps, epsi = (T)c.epsi;
+            const T dxi = q.dxi, ddxi = q.ddxi;
+            for (int e = tid; e < PFX * PFY; e += PTHREADS) {
+                int fx_ = e % PFX, fy_ = e / PFX;
+                int pc = (fy_ + 1) * PBX + (fx_ + 1);
+                T f0 = PL[ps][pc];
+                T fxp = PL[ps][pc + 1], fxm = PL[ps][pc - 1];
+                T fyp = PL[ps][pc + PBX], fym = PL[ps][pc - PBX];
+                T fzp = PL[psp][pc], fzm = PL[psm][pc];
+                T t1 = f0 * ((T)1.0 - f0) * ((T)1.0 - (T)2.0 * f0) * epsi;
+                T s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - (T)6.0 * f0;
+                T chem = t1 - eps * s6 * ddxi;
+                FL[fs][0][e] = sig6 * chem * ((T)0.5 * (fxp - fxm) * dxi);
+                FL[fs][1][e] = sig6 * chem * ((T)0.5 * (fyp - fym) * dxi);
+                FL[fs][2][e] = sig6 * chem * ((T)0.5 * (fzp - fzm) * dxi);
+            }
+        }
+    };
+
+    load_u(k0 - 1);
+    load_u(k0);
+    load_phi(k0 - 2);
+    load_phi(k0 - 1);
+    load_phi(k0);
+    fetch_u(k0 + 1);
+    fetch_phi(k0 + 1);
+    __syncthreads();
+    force(k0 - 1);
+    __syncthreads();
+
+    const int tx = tid % PTX, ty = tid / PTX;
+    const int i = i0 + 2 * tx, j = j0 + ty;
+    const bool active = (i < n) && (j < n);        // n even: i+1 < n with i
+    const int kend = min(k0 + kz, n);
+    const int uc = (ty + 1) * PBX + (2 * tx + 2);
+    const int fc = (ty + 1) * PFX + (2 * tx + 1);
+    const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
+    const float sj = (float)sinf[j < n ? j : 0], cj = (float)cosf[j < n ? j : 0];
+    const int ia = i < n ? i : 0, ib = i + 1 < n ? i + 1 : 0;
+    const float sia = (float)sinf[ia], cia = (float)cosf[ia], sib = (float)sinf[ib], cib = (float)cosf[ib];
+    float2 ro[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (active) {
+        ro[0] = ld2(ruo + cxy + nn * (size_t)k0);
+        ro[1] = ld2(rvo + cxy + nn * (size_t)k0);
+        ro[2] = ld2(rwo + cxy + nn * (size_t)k0);
+    }
+    for (int k = k0; k < kend; k++) {
+        store_phi(k + 1);
+        if (k + 1 < kend) fetch_phi(k + 2);
+        __syncthreads();
+        force(k);
+        store_u(k + 1);
+        if (k + 1 < kend) fetch_u(k + 2);
+        const float2 r0 = ro[0], r1 = ro[1], r2 = ro[2];
+        if (active && k + 1 < kend) {
+            ro[0] = ld2(ruo + cxy + nn * (size_t)(k + 1));
+            ro[1] = ld2(rvo + cxy + nn * (size_t)(k + 1));
+            ro[2] = ld2(rwo + cxy + nn * (size_t)(k + 1));
+        }
+        __syncthreads();
+        if (active) {
+            const int s0 = (k - k0 + 1) % 3, sm = (k - k0) % 3, sp = (k - k0 + 2) % 3;
+            const int fs = (k - k0 + 1) & 1, fsm = (k - k0) & 1;
+            const T *Um = UL[sm][0], *Vm = UL[sm][1], *Wm = UL[sm][2];
+            const float sk = (float)sinf[k], ck = (float)cosf[k];
+            float oa[6], ob[6];
+            mom_point<PHI, PBX, PFX>(UL[s0][0], UL[s0][1], UL[s0][2], UL[sp][0], UL[sp][1], UL[sp][2], uc,
+                                     Um[uc], Vm[uc], Wm[uc], Um[uc + 1], Vm[uc + PBX], FL[fs][0], FL[fs][1],
+                                     FL[fs][2], PHI ? FL[fsm][2][fc] : 0.0f, fc, sk, cj, sia, ck, sj, cia, r0.x,
+                                     r1.x, r2.x, q, oa);
+            mom_point<PHI, PBX, PFX>(UL[s0][0], UL[s0][1], UL[s0][2], UL[sp][0], UL[sp][1], UL[sp][2], uc + 1,
+                                     Um[uc + 1], Vm[uc + 1], Wm[uc + 1], Um[uc + 2], Vm[uc + 1 + PBX], FL[fs][0],
+                                     FL[fs][1], FL[fs][2], PHI ? FL[fsm][2][fc + 1] : 0.0f, fc + 1, sk, cj, sib, ck,
+                                     sj, cib, r0.y, r1.y, r2.y, q, ob);
+            const size_t t = cxy + nn * (size_t)k;
+            st2nt(us + t, oa[0], ob[0]);
+            st2nt(vs + t, oa[1], ob[1]);
+            st2nt(ws + t, oa[2], ob[2]);
+            st2nt(ruo + t, oa[3], ob[3]);
+            st2nt(rvo + t, oa[4], ob[4]);
+            st2nt(rwo + t, oa[5], ob[5]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FFT machinery: mixed-radix Stockham autosort in LDS (Govindaraju et al.,
 // SC'08), unnormalised.  A block holds `nlines` lines of length n; element p of
 // line l sits at lds[l*lstride + p*pstride].  Each thread owns at most one
@@ -1360,11 +1621,12 @@ void launch_momentum(const Fields &f, const Consts &c, const Tuning &tu, double 
         using T = float;
         const T *u = (const T *)f.u, *v = (const T *)f.v, *w = (const T *)f.w, *ph = (const T *)f.phi;
         T *us = (T *)f.us, *vs = (T *)f.vs, *ws = (T *)f.ws, *ru = (T *)f.ruo, *rv = (T *)f.rvo, *rw = (T *)f.rwo;
+        const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, n / kz);
         if (c.phiflag)
-            hipLaunchKernelGGL((k_momentum<1, T>), grid, dim3(MTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+            hipLaunchKernelGGL((k_momentum_x2<1>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
                                f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
         else
-            hipLaunchKernelGGL((k_momentum<0, T>), grid, dim3(MTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+            hipLaunchKernelGGL((k_momentum_x2<0>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
                                f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
     } else {
         if (c.phiflag)
