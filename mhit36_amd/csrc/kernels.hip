@@ -464,12 +464,13 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
 }
 
 // ---------------------------------------------------------------------------
-// The same pass for FP32 state with two x-points per thread: 64 x 8 points
+// The same pass with two x-points per thread, used for FP32 state (in FP64
+// the pair needs 220 VGPRs and 95 KB of LDS: one wave per SIMD): 64 x 8 points
 // per block, every global load and store a float2 (the u/v/w and phi boxes
 // start at the even column i0-2, so pairs never straddle the periodic seam),
 // half the memory instructions per byte of the one-point form.  Each point's
 // arithmetic is mom_point below, written exactly as in k_momentum, so the
-// results are those of k_momentum<PHI, float> bit for bit.
+// results are those of k_momentum<PHI, T> bit for bit.
 // ---------------------------------------------------------------------------
 constexpr int PTX = 32, PTY = 8, PTHREADS = PTX * PTY;   // threads; 2 points each in x
 constexpr int PBX = 2 * PTX + 4, PUY = PTY + 2;           // u/v/w box row (68) and rows (10)
@@ -478,22 +479,21 @@ constexpr int PFX = 2 * PTX + 1, PFY = PTY + 1;           // force box (65 x 9)
 static_assert((PBX / 2) * PUY <= 2 * PTHREADS && (PBX / 2) * PPY <= 2 * PTHREADS, "two pairs per thread");
 static_assert(PFX * PFY <= 3 * PTHREADS, "three force points per thread");
 
+template <typename T>
 struct MomK {
-    float dxi, ddxi, mu, rhoi, f1, f2, f3, dt, alpha, beta;
+    T dxi, ddxi, mu, rhoi, f1, f2, f3, dt, alpha, beta;
 };
 
 // A11-A15 at one point, main.f90:513-696 (as in k_momentum).  U0..W0: plane
 // k, Up..Wp: plane k+1, with row stride SX; uzm..wzm, uxpm (u at k-1, i+1)
 // and vypm (v at k-1, j+1) from plane k-1; FX/FY/FZ the force of plane k and
 // fzm its z part at plane k-1 (row stride FXS).
-template <int PHI, int SX, int FXS>
-__device__ __forceinline__ void mom_point(const float *U0, const float *V0, const float *W0, const float *Up,
-                                          const float *Vp, const float *Wp, int uc, float uzm, float vzm, float wzm,
-                                          float uxpm, float vypm, const float *FX, const float *FY, const float *FZ,
-                                          float fzm, int fc, float sk, float cj, float si, float ck, float sj, float ci,
-                                          float ro0, float ro1, float ro2, const MomK &q, float *out)
+template <int PHI, int SX, int FXS, typename T>
+__device__ __forceinline__ void mom_point(const T *U0, const T *V0, const T *W0, const T *Up, const T *Vp,
+                                          const T *Wp, int uc, T uzm, T vzm, T wzm, T uxpm, T vypm, const T *FX,
+                                          const T *FY, const T *FZ, T fzm, int fc, T sk, T cj, T si, T ck, T sj, T ci,
+                                          T ro0, T ro1, T ro2, const MomK<T> &q, T *out)
 {
-    using T = float;
     const T dxi = q.dxi, ddxi = q.ddxi, mu = q.mu, rhoi = q.rhoi, f1 = q.f1, f2 = q.f2, f3 = q.f3;
     const T u0 = U0[uc], v0 = V0[uc], w0 = W0[uc];
     const T uxp = U0[uc + 1], uxm = U0[uc - 1], uyp = U0[uc + SX], uym = U0[uc - SX];
@@ -546,23 +546,32 @@ __device__ __forceinline__ void mom_point(const float *U0, const float *V0, cons
     out[5] = rw;
 }
 
-__device__ __forceinline__ float2 ld2(const float *p) { return *reinterpret_cast<const float2 *>(p); }
-__device__ __forceinline__ void st2nt(float *p, float a, float b)
+template <typename T> struct Pair2;
+template <> struct Pair2<float> { typedef float2 type; };
+template <> struct Pair2<double> { typedef double2 type; };
+
+template <typename T>
+__device__ __forceinline__ typename Pair2<T>::type ld2(const T *p)
 {
-    typedef float nt_float2 __attribute__((ext_vector_type(2)));
-    nt_float2 v = {a, b};
-    __builtin_nontemporal_store(v, reinterpret_cast<nt_float2 *>(p));
+    return *reinterpret_cast<const typename Pair2<T>::type *>(p);
+}
+template <typename T>
+__device__ __forceinline__ void st2nt(T *p, T a, T b)
+{
+    typedef T nt_pair __attribute__((ext_vector_type(2)));
+    nt_pair v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_pair *>(p));
 }
 
-template <int PHI>
+template <int PHI, typename T>
 __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
-    const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
-    const float *__restrict__ phi, float *__restrict__ us, float *__restrict__ vs, float *__restrict__ ws,
-    float *__restrict__ ruo, float *__restrict__ rvo, float *__restrict__ rwo,
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi, T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
+    T *__restrict__ ruo, T *__restrict__ rvo, T *__restrict__ rwo,
     const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
     Consts c, double ALPHA, double BETA, int kz)
 {
-    using T = float;
+    using P2 = typename Pair2<T>::type;
     if (halted(sc)) return;
     __shared__ T UL[3][3][PUY * PBX];
     __shared__ T PL[PHI ? 3 : 1][PHI ? PPY * PBX : 1];
@@ -572,7 +581,7 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
     const uint3 blk = xcd_block();
     const int i0 = blk.x * (2 * PTX), j0 = blk.y * PTY, k0 = blk.z * kz;
     const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
-    MomK q;
+    MomK<T> q;
     q.dxi = (T)c.dxi; q.ddxi = (T)c.ddxi; q.mu = (T)c.mu; q.rhoi = (T)c.rhoi;
     q.f1 = (T)c.f1; q.f2 = (T)c.f2; q.f3 = (T)c.f3;
     q.dt = (T)c.dt; q.alpha = (T)ALPHA; q.beta = (T)BETA;
@@ -588,7 +597,7 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
         foff[r] = (unsigned)x + (unsigned)n * wrap(j0 - 2 + e / NP, n);
     }
     const size_t nn = (size_t)n * n;
-    float2 pu[2][3], pp[2];
+    P2 pu[2][3], pp[2];
     auto fetch_u = [&](int m) {
         const size_t zoff = nn * (size_t)wrap(m, n);
 #pragma unroll
@@ -676,10 +685,11 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
     const int uc = (ty + 1) * PBX + (2 * tx + 2);
     const int fc = (ty + 1) * PFX + (2 * tx + 1);
     const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
-    const float sj = (float)sinf[j < n ? j : 0], cj = (float)cosf[j < n ? j : 0];
+    const T sj = (T)sinf[j < n ? j : 0], cj = (T)cosf[j < n ? j : 0];
     const int ia = i < n ? i : 0, ib = i + 1 < n ? i + 1 : 0;
-    const float sia = (float)sinf[ia], cia = (float)cosf[ia], sib = (float)sinf[ib], cib = (float)cosf[ib];
-    float2 ro[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    const T sia = (T)sinf[ia], cia = (T)cosf[ia], sib = (T)sinf[ib], cib = (T)cosf[ib];
+    P2 ro[3];
+    ro[0].x = ro[0].y = ro[1].x = ro[1].y = ro[2].x = ro[2].y = (T)0.0;
     if (active) {
         ro[0] = ld2(ruo + cxy + nn * (size_t)k0);
         ro[1] = ld2(rvo + cxy + nn * (size_t)k0);
@@ -692,7 +702,7 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
         force(k);
         store_u(k + 1);
         if (k + 1 < kend) fetch_u(k + 2);
-        const float2 r0 = ro[0], r1 = ro[1], r2 = ro[2];
+        const P2 r0 = ro[0], r1 = ro[1], r2 = ro[2];
         if (active && k + 1 < kend) {
             ro[0] = ld2(ruo + cxy + nn * (size_t)(k + 1));
             ro[1] = ld2(rvo + cxy + nn * (size_t)(k + 1));
@@ -703,15 +713,15 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
             const int s0 = (k - k0 + 1) % 3, sm = (k - k0) % 3, sp = (k - k0 + 2) % 3;
             const int fs = (k - k0 + 1) & 1, fsm = (k - k0) & 1;
             const T *Um = UL[sm][0], *Vm = UL[sm][1], *Wm = UL[sm][2];
-            const float sk = (float)sinf[k], ck = (float)cosf[k];
-            float oa[6], ob[6];
+            const T sk = (T)sinf[k], ck = (T)cosf[k];
+            T oa[6], ob[6];
             mom_point<PHI, PBX, PFX>(UL[s0][0], UL[s0][1], UL[s0][2], UL[sp][0], UL[sp][1], UL[sp][2], uc,
                                      Um[uc], Vm[uc], Wm[uc], Um[uc + 1], Vm[uc + PBX], FL[fs][0], FL[fs][1],
-                                     FL[fs][2], PHI ? FL[fsm][2][fc] : 0.0f, fc, sk, cj, sia, ck, sj, cia, r0.x,
+                                     FL[fs][2], PHI ? FL[fsm][2][fc] : (T)0.0, fc, sk, cj, sia, ck, sj, cia, r0.x,
                                      r1.x, r2.x, q, oa);
             mom_point<PHI, PBX, PFX>(UL[s0][0], UL[s0][1], UL[s0][2], UL[sp][0], UL[sp][1], UL[sp][2], uc + 1,
                                      Um[uc + 1], Vm[uc + 1], Wm[uc + 1], Um[uc + 2], Vm[uc + 1 + PBX], FL[fs][0],
-                                     FL[fs][1], FL[fs][2], PHI ? FL[fsm][2][fc + 1] : 0.0f, fc + 1, sk, cj, sib, ck,
+                                     FL[fs][1], FL[fs][2], PHI ? FL[fsm][2][fc + 1] : (T)0.0, fc + 1, sk, cj, sib, ck,
                                      sj, cib, r0.y, r1.y, r2.y, q, ob);
             const size_t t = cxy + nn * (size_t)k;
             st2nt(us + t, oa[0], ob[0]);
@@ -1623,10 +1633,10 @@ void launch_momentum(const Fields &f, const Consts &c, const Tuning &tu, double 
         T *us = (T *)f.us, *vs = (T *)f.vs, *ws = (T *)f.ws, *ru = (T *)f.ruo, *rv = (T *)f.rvo, *rw = (T *)f.rwo;
         const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, n / kz);
         if (c.phiflag)
-            hipLaunchKernelGGL((k_momentum_x2<1>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+            hipLaunchKernelGGL((k_momentum_x2<1, T>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
                                f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
         else
-            hipLaunchKernelGGL((k_momentum_x2<0>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
+            hipLaunchKernelGGL((k_momentum_x2<0, T>), g2, dim3(PTHREADS), 0, s, u, v, w, ph, us, vs, ws, ru, rv, rw,
                                f.sinf, f.cosf, f.sc, c, alpha, beta, kz);
     } else {
         if (c.phiflag)
