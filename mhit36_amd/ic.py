@@ -97,7 +97,7 @@ def drops(n, centres, radius, eps):
     lx = 8.0 * np.arctan(1.0)
     phi = np.zeros((n, n, n), order="F")
     half = radius + 40 * eps      # profile below 1e-17 beyond: no jump at the box edge
-    w = int(np.ceil(half / dx)) + 1
+    w = min(int(np.ceil(half / dx)) + 1, n // 2)     # beyond n/2 the box only repeats cells
     for c in centres:
         idx = []
         dist = []
