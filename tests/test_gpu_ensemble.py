@@ -84,3 +84,40 @@ def test_submitted_fields_are_used():
         rid = ens.submit(default_params(**p), nsteps=5, u=u, v=v, w=w)
         ens.wait_idle()
         assert np.array_equal(ens.get_field(rid, 1), ref)
+
+
+def test_close_with_boxes_in_flight():
+    """Destroying the ensemble while boxes are admitted, stepping and queued
+    returns cleanly (I/O threads joined, streams drained)."""
+    p = dict(nx=64, dt=5e-4, mu=0.005, phiflag=1, radius=0.8, sigma=0.1, precision=1)
+    ens = Ensemble(max_active=2, inflight=3)
+    for _ in range(5):
+        ens.submit(default_params(**p), nsteps=500)
+    for _ in range(20):
+        ens.progress()
+    ens.close()
+
+
+def test_results_arrive_while_others_run():
+    """Continuous batching: a short box finishes, and a box submitted later is
+    admitted, while a long box keeps running."""
+    long_p = dict(nx=96, dt=5e-4, mu=0.005, phiflag=0, precision=0)
+    short_p = dict(nx=32, dt=1e-3, mu=0.01, phiflag=0, precision=0)
+    with Ensemble(inflight=2) as ens:
+        lid = ens.submit(default_params(**long_p), nsteps=400)
+        sid = ens.submit(default_params(**short_p), nsteps=5)
+        done = {}
+        late = None
+        while True:
+            act, q = ens.progress()
+            for r in ens.results():
+                done[r["id"]] = r
+            if sid in done and late is None:
+                assert lid not in done            # the long box is still running
+                late = ens.submit(default_params(**short_p), nsteps=5)
+            if act == 0 and q == 0 and late is not None:
+                break
+        for r in ens.results():
+            done[r["id"]] = r
+        assert done[late]["admit_ms"] < done[lid]["finish_ms"]
+        assert np.array_equal(ens.get_field(late, 1), ens.get_field(sid, 1))
