@@ -274,7 +274,13 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     if (halted(sc)) return;
     __shared__ T UL[3][3][MUY * MUX];
     __shared__ T PL[PHI ? 3 : 1][PHI ? MPY * MPX : 1];
-    __shared__ T FL[PHI ? 2 : 1][3][PHI ? MFY * MFX : 1];
+    // CSF force of plane k, only where the face interpolation reads it: the x
+    // part at x = -1..MTX-1, the y part at y = -1..MTY-1, the z part at the
+    // points themselves (tile coordinates); the z part of plane k-1 at the
+    // thread's own point is carried in a register (fzm).
+    __shared__ T FXL[PHI ? MTY * (MTX + 1) : 1];
+    __shared__ T FYL[PHI ? (MTY + 1) * MTX : 1];
+    __shared__ T FZL[PHI ? MTY * MTX : 1];
     const int n = c.n;
     const int tid = threadIdx.x;
     const uint3 blk = xcd_block();
@@ -340,10 +346,10 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     auto force = [&](int m) {           // A14 first half on plane m
         if constexpr (PHI) {
             const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
-            const int fs = (m - k0 + 1) & 1;
             const T sig6 = (T)((T)6.0 * c.sigma), eps = (T)c.eps, epsi = (T)c.epsi;
             for (int e = tid; e < MFX * MFY; e += MTHREADS) {
-                int fx_ = e % MFX, fy_ = e / MFX;
+                int fx_ = e % MFX, fy_ = e / MFX;              // cell x = fx_-1, y = fy_-1 in the tile
+                if (fx_ == 0 && fy_ == 0) continue;             // the corner cell is never read
                 int pc = (fy_ + 1) * MPX + (fx_ + 1);
                 T f0 = PL[ps][pc];
                 T fxp = PL[ps][pc + 1], fxm = PL[ps][pc - 1];
@@ -352,9 +358,9 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
                 T t1 = f0 * ((T)1.0 - f0) * ((T)1.0 - (T)2.0 * f0) * epsi;
                 T s6 = (((((fxp + fxm) + fyp) + fym) + fzp) + fzm) - (T)6.0 * f0;
                 T chem = t1 - eps * s6 * ddxi;
-                FL[fs][0][e] = sig6 * chem * ((T)0.5 * (fxp - fxm) * dxi);
-                FL[fs][1][e] = sig6 * chem * ((T)0.5 * (fyp - fym) * dxi);
-                FL[fs][2][e] = sig6 * chem * ((T)0.5 * (fzp - fzm) * dxi);
+                if (fy_ > 0) FXL[(fy_ - 1) * MFX + fx_] = sig6 * chem * ((T)0.5 * (fxp - fxm) * dxi);
+                if (fx_ > 0) FYL[fy_ * MTX + (fx_ - 1)] = sig6 * chem * ((T)0.5 * (fyp - fym) * dxi);
+                if (fx_ > 0 && fy_ > 0) FZL[(fy_ - 1) * MTX + (fx_ - 1)] = sig6 * chem * ((T)0.5 * (fzp - fzm) * dxi);
             }
         }
     };
@@ -377,7 +383,11 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     const bool active = (i < n) && (j < n);
     const int kend = min(k0 + kz, n);
     const int uc = (ty + 1) * MUX + (tx + 1);
-    const int fc = (ty + 1) * MFX + (tx + 1);
+    const int fxc = ty * MFX + (tx + 1);       // x part at x = tx (FXL[fxc-1]: x = tx-1)
+    const int fyc = (ty + 1) * MTX + tx;       // y part at y = ty (FYL[fyc-MTX]: y = ty-1)
+    const int fzc = ty * MTX + tx;
+    T fzm = (T)0.0;                            // z part of the force at plane k-1, this point
+    if constexpr (PHI) fzm = FZL[fzc];
     const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
     T ro[3] = {(T)0.0, (T)0.0, (T)0.0};
     if (active) {
@@ -385,10 +395,11 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
         ro[1] = rvo[cxy + nn * (size_t)k0];
         ro[2] = rwo[cxy + nn * (size_t)k0];
     }
-    // Ring slots (3 each): u plane m in (m-k0+1)%3, phi plane m in (m-k0+2)%3,
-    // force plane m in (m-k0+1)%2.  phi(k+1) replaces phi(k-2), last read by
-    // force(k-1) before the previous iteration's second barrier; u(k+1)
-    // replaces u(k-2), last read by output(k-1) before this iteration's first.
+    // Ring slots (3 each): u plane m in (m-k0+1)%3, phi plane m in (m-k0+2)%3.
+    // phi(k+1) replaces phi(k-2), last read by force(k-1) before the previous
+    // iteration's second barrier; u(k+1) replaces u(k-2), and force(k) the
+    // force of plane k-1, both last read by output(k-1) before this
+    // iteration's first barrier.
     for (int k = k0; k < kend; k++) {
         store_phi(k + 1);
         if (k + 1 < kend) fetch_phi(k + 2);
@@ -446,10 +457,11 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
             rv = (rv + f1 * (T)sinf[i]) + f3 * (T)cosf[k];
             rw = (rw + f2 * (T)sinf[j]) + f1 * (T)cosf[i];
             if constexpr (PHI) {
-                const int fs = (k - k0 + 1) & 1, fsm = (k - k0) & 1;
-                ru = ru + (T)0.5 * (FL[fs][0][fc - 1] + FL[fs][0][fc]) * rhoi;
-                rv = rv + (T)0.5 * (FL[fs][1][fc - MFX] + FL[fs][1][fc]) * rhoi;
-                rw = rw + (T)0.5 * (FL[fsm][2][fc] + FL[fs][2][fc]) * rhoi;
+                const T fz0 = FZL[fzc];
+                ru = ru + (T)0.5 * (FXL[fxc - 1] + FXL[fxc]) * rhoi;
+                rv = rv + (T)0.5 * (FYL[fyc - MTX] + FYL[fyc]) * rhoi;
+                rw = rw + (T)0.5 * (fzm + fz0) * rhoi;
+                fzm = fz0;
             }
             const T dt = (T)c.dt, alpha = (T)ALPHA, beta = (T)BETA;
             const size_t t = cxy + nn * (size_t)k;
