@@ -158,6 +158,16 @@ int mhit_kernel_stats(mhit_ctx *ctx, mhit_kstat *out, int max_entries, int *n_en
 /* Device memory held by the context, bytes. */
 int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
 
+/* Checkpoint and resume of the exact device state (no counterpart in the
+ * reference, whose restart re-reads u, v, w, phi and restarts AB2 with an
+ * Euler step, main.f90:243-245, 323-326): u, v, w as stored with their pending
+ * means, rhs_o, phi, p, the scalars (umax behind the lagged gamma, halt
+ * state), the time index and the AB2 state.  A context loaded from a state
+ * file continues bit for bit as if it had not stopped.  load needs a context
+ * of the same nx, precision and phiflag (MHIT_ERR_ARG otherwise). */
+int mhit_save_state(mhit_ctx *ctx, const char *path);
+int mhit_load_state(mhit_ctx *ctx, const char *path);
+
 /* Launch-shape autotuner (no counterpart in the reference, whose kernel
  * shapes are fixed by nvfortran): times the candidate launch shapes of the
  * tunable passes -- z planes per block of the phase-field and momentum passes,

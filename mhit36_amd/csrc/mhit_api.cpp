@@ -752,6 +752,102 @@ extern "C" int mhit_read_field(mhit_ctx *c, const char *path, int fieldn)
 }
 
 // ---------------------------------------------------------------------------
+// Checkpoint of the exact device state (no counterpart in the reference, whose
+// restart re-reads u, v, w, phi and restarts AB2 with an Euler step): the
+// stored u, v, w with their pending means, rhs_o, phi and p as they sit in
+// memory (FP32 or FP64), the scalars (means, umax, halt flag, step count),
+// the time index and the AB2 state, so that a loaded context continues bit
+// for bit as if it had never stopped.
+// ---------------------------------------------------------------------------
+namespace {
+struct StateHeader {
+    char magic[8];                 // "MHITSTAT"
+    int abi, nx, precision, phiflag, t, nsteps_since_init, halted, halt_status;
+    Scalars sc;
+};
+}  // namespace
+
+static std::vector<std::pair<double *, size_t>> state_buffers(mhit_ctx *c)
+{
+    const size_t n3 = (size_t)c->kc.n * c->kc.n * c->kc.n;
+    const size_t FT = n3 * (c->kc.mixed ? sizeof(float) : sizeof(double));
+    std::vector<std::pair<double *, size_t>> b = {{c->f.u, FT},   {c->f.v, FT},   {c->f.w, FT},
+                                                  {c->f.ruo, FT}, {c->f.rvo, FT}, {c->f.rwo, FT},
+                                                  {c->f.p, n3 * sizeof(double)}};
+    if (c->kc.phiflag) b.push_back({c->f.phi, FT});
+    return b;
+}
+
+extern "C" int mhit_save_state(mhit_ctx *c, const char *path)
+{
+    if (!c || !path) return MHIT_ERR_ARG;
+    StateHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, "MHITSTAT", 8);
+    h.abi = MHIT_ABI_VERSION;
+    h.nx = c->kc.n;
+    h.precision = c->kc.mixed ? MHIT_MIXED : MHIT_FP64;
+    h.phiflag = c->kc.phiflag;
+    h.t = c->t;
+    h.nsteps_since_init = c->nsteps_since_init;
+    h.halted = c->halted;
+    h.halt_status = c->halt_status;
+    int rc = read_scalars(c, &h.sc);
+    if (rc) return rc;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return MHIT_ERR_IO;
+    bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
+    std::vector<char> buf;
+    for (auto &b : state_buffers(c)) {
+        if (!ok) break;
+        buf.resize(b.second);
+        ok = hipMemcpyAsync(buf.data(), b.first, b.second, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+        if (!ok) {
+            fclose(fp);
+            return MHIT_ERR_HIP;
+        }
+        ok = fwrite(buf.data(), 1, b.second, fp) == b.second;
+    }
+    const int cl = fclose(fp);
+    return (ok && cl == 0) ? MHIT_OK : MHIT_ERR_IO;
+}
+
+extern "C" int mhit_load_state(mhit_ctx *c, const char *path)
+{
+    if (!c || !path) return MHIT_ERR_ARG;
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return MHIT_ERR_IO;
+    StateHeader h;
+    if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "MHITSTAT", 8) != 0) {
+        fclose(fp);
+        return MHIT_ERR_IO;
+    }
+    if (h.nx != c->kc.n || h.phiflag != c->kc.phiflag || h.precision != (c->kc.mixed ? MHIT_MIXED : MHIT_FP64)) {
+        fclose(fp);
+        return MHIT_ERR_ARG;                                  // a state of another kind of box
+    }
+    std::vector<char> buf;
+    int rc = MHIT_OK;
+    for (auto &b : state_buffers(c)) {
+        buf.resize(b.second);
+        if (fread(buf.data(), 1, b.second, fp) != b.second) { rc = MHIT_ERR_IO; break; }
+        if (hipMemcpyAsync(b.first, buf.data(), b.second, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = MHIT_ERR_HIP; break; }
+    }
+    fclose(fp);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->f.sc, &h.sc, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->t = h.t;
+    c->nsteps_since_init = h.nsteps_since_init;
+    c->halted = h.halted;
+    c->halt_status = h.halt_status;
+    c->loaded = true;
+    return MHIT_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Launch-shape autotuner
 // ---------------------------------------------------------------------------
 namespace {

@@ -97,6 +97,16 @@ module mhit_c
        integer(c_int), intent(out) :: n_drops
        real(c_double), dimension(*), intent(out) :: volume, deq
      end function
+     integer(c_int) function mhit_save_state(ctx, path) bind(C, name="mhit_save_state")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), dimension(*), intent(in) :: path
+     end function
+     integer(c_int) function mhit_load_state(ctx, path) bind(C, name="mhit_load_state")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), dimension(*), intent(in) :: path
+     end function
      type(c_ptr) function mhit_strerror_c(code) bind(C, name="mhit_strerror")
        import :: c_int, c_ptr
        integer(c_int), value :: code

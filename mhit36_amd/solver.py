@@ -145,6 +145,14 @@ class Solver:
         check(self._lib.mhit_diagnostics(self._h, float(dxi), float(nu), ctypes.byref(st)), "diagnostics")
         return {name: getattr(st, name) for name, _ in Stats._fields_}
 
+    def save_state(self, path):
+        """Checkpoint the exact device state (mhit_save_state)."""
+        check(self._lib.mhit_save_state(self._h, str(path).encode()), "save_state")
+
+    def load_state(self, path):
+        """Resume from mhit_save_state's file: continues bit for bit."""
+        check(self._lib.mhit_load_state(self._h, str(path).encode()), "load_state")
+
     def drop_count(self, max_drops=1 << 16):
         """post/DropCountSize on the current phi: number of drops, and their
         volumes (cells) and equivalent diameters in raster order of each
