@@ -13,6 +13,8 @@
   phase-field update goes unstable near CFL 0.14 once forcing raises umax.
 - 512^3 NS+ACDI with 512 drops of radius 0.1 (the benchmark box): bit-identical
   to the oracle over two steps.
+- 192^3 with 16 drops (a radix-3 size of the serving stream, LDS-staged FFT):
+  bit-identical over three steps; mixed within 1e-5 of FP64 over 50 steps.
 - 1024^3 NS-only (Taylor-Green start with ABC forcing, nu = 0.001; the random
   spectrum is not generated at this size on the host) resident on one GPU:
   FP64 against mixed precision over 20 steps.
@@ -127,3 +129,35 @@ def test_c4a_1024_ns_resident():
     assert abs(res[1][0] - res[0][0]) <= 1e-5 * res[0][0]
     assert abs(res[1][1]["diss"] - res[0][1]["diss"]) <= 1e-5 * res[0][1]["diss"]
     assert abs(res[1][1]["ke"] - res[0][1]["ke"]) <= 1e-6 * res[0][1]["ke"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_192_radix3_acdi(precision):
+    """A radix-3 size of the serving stream (192 = 2^6 3: the LDS-staged FFT
+    kernels): FP64 bit-identical to the oracle over three steps; mixed within
+    1e-5 of FP64 over 50 steps."""
+    n = 192
+    u, v, w, dx, umax = hit(n, seed=77)
+    phi = ic.drops(n, ic.drop_centres(16, 0.3, dx, seed=5), 0.3, dx)
+    kw = dict(nx=n, dt=0.05 * dx / umax, mu=0.004, f1=1.0, f2=1.0, f3=1.0, phiflag=1, inflow=1, inphi=1,
+              radius=0.3, sigma=0.1, gamma_factor=1.2)
+    fields = {1: u, 2: v, 3: w, 5: phi}
+    if precision == 0:
+        with Solver(**kw) as s:
+            load(s, fields)
+            o = orc.Oracle(**kw)
+            load(o, fields)
+            for _ in range(3):
+                s.step(1)
+                o.step(1)
+                assert_same(s, o, (1, 2, 3, 4, 5))
+            o.close()
+    else:
+        res = {}
+        for prec in (0, 1):
+            with Solver(**kw, precision=prec) as s:
+                load(s, fields)
+                s.step(50)
+                res[prec] = s.diagnostics()
+        assert abs(res[1]["diss"] - res[0]["diss"]) <= 1e-5 * res[0]["diss"]
+        assert abs(res[1]["phi_sum"] - res[0]["phi_sum"]) <= 1e-6 * res[0]["phi_sum"]
