@@ -137,6 +137,19 @@ int mhit_get_field(mhit_ctx *ctx, int fieldn, double *host);
  * (MHIT_ERR_CFL) or when u, v or w holds a NaN (MHIT_ERR_NAN). */
 int mhit_step(mhit_ctx *ctx, int nsteps, mhit_diag *diag);
 
+/* Step several boxes together (no counterpart in the reference, which runs one
+ * box per job): ctx[0..nbox) are distinct contexts of one nx, precision and
+ * phiflag (MHIT_ERR_ARG otherwise), at most MHIT_BATCH_MAX of them.  Each pass
+ * of a step is one launch for all boxes, so small boxes fill the GPU; every
+ * box's fields are bit-identical to mhit_step on that box alone.  Boxes may
+ * differ in every other parameter and in their step count (an Euler first
+ * step beside AB2 steps).  diag[b] (may be NULL) is box b's mhit_step result;
+ * a box that stops (CFL, NaN) or was already stopped does no further steps
+ * while the others go on.  Returns MHIT_OK, or the first box's error.  The
+ * launches go on ctx[0]'s stream; returns after the device work completes. */
+#define MHIT_BATCH_MAX 16
+int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_diag *diag);
+
 /* Set the step counter and make the next step an Euler step (restart). */
 int mhit_reset_time(mhit_ctx *ctx, int tstart);
 

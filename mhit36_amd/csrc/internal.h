@@ -75,6 +75,15 @@ struct Fields {
     int ld;                      // spectrum leading dimension (>= n/2+1)
 };
 
+// One box of a gang: the boxes of one n, precision and phiflag that a single
+// launch per pass steps together.  An array of these sits in device memory;
+// phi/phin are as the step being launched reads and writes them.
+constexpr int GANG_MAX = 16;
+struct GangBox {
+    Fields f;
+    Consts c;
+};
+
 // Launch helpers (kernels.hip).  Each returns the name and the algorithmic
 // bytes of the launch through `info` so that the host can time it.
 struct LaunchInfo {
@@ -106,6 +115,10 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
+// One step of nbox boxes (descriptors g in device memory, each of c's n,
+// precision and phiflag; bit b of euler: box b takes its Euler first step).
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &c, const FftPlan &pl, int ld,
+                      const Tuning &tu, hipStream_t s);
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s);
 // Drop counting: label the phi >= 0.5 cells (26-connected, periodic) by the
 // smallest linear index of their set; list the roots; count cells per root

@@ -72,10 +72,8 @@ __device__ __forceinline__ float ttanh(float x, const double *)
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tdiv(float a, float b) { return __fdividef(a, b); }
 
-// FP32: 8 waves per SIMD (64 VGPRs, a few spilled) measured 12% faster than
-// the 7 the register count would give; FP64 is held at 4 by its LDS rings.
 template <typename T>
-__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
+__device__ __forceinline__ void k_acdi_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, const double *__restrict__ mtab,
     Consts c, int kz)
@@ -89,7 +87,6 @@ __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
     __shared__ T phL[4][ASY * ASX];
     const int n = c.n;
     const int tid = threadIdx.x;
-    const uint3 blk = xcd_block();
     const int i0 = blk.x * ATX, j0 = blk.y * ATY, k0 = blk.z * kz;
     const T eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
     const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
@@ -244,6 +241,18 @@ __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
     }
 }
 
+// FP32: 8 waves per SIMD (64 VGPRs, a few spilled) measured 12% faster than
+// the 7 the register count would give; FP64 is held at 4 by its LDS rings.
+template <typename T>
+__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, const double *__restrict__ mtab,
+    Consts c, int kz)
+{
+    k_acdi_body<T>(xcd_block(), u, v, w, phi, phin, sc, mtab, c, kz);
+}
+
+
 // ---------------------------------------------------------------------------
 // Momentum, main.f90:513-681: A11 convection, A12 viscous, A13 ABC forcing,
 // A14 CSF force (phi^{n+1}) interpolated to the faces, A15 AB2/Euler advance
@@ -263,7 +272,7 @@ constexpr int MFX = MTX + 1, MFY = MTY + 1;       // force box
 static_assert(MUX * MUY <= 2 * MTHREADS && MPX * MPY <= 2 * MTHREADS, "box loads assume <= 2 elements per thread");
 
 template <int PHI, typename T>
-__global__ void __launch_bounds__(MTHREADS) k_momentum(
+__device__ __forceinline__ void k_momentum_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ phi,
     T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
@@ -283,7 +292,6 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     __shared__ T FZL[PHI ? MTY * MTX : 1];
     const int n = c.n;
     const int tid = threadIdx.x;
-    const uint3 blk = xcd_block();
     const int i0 = blk.x * MTX, j0 = blk.y * MTY, k0 = blk.z * kz;
     const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const T dxi = (T)c.dxi, ddxi = (T)c.ddxi, mu = (T)c.mu, rhoi = (T)c.rhoi;
@@ -475,6 +483,19 @@ __global__ void __launch_bounds__(MTHREADS) k_momentum(
     }
 }
 
+template <int PHI, typename T>
+__global__ void __launch_bounds__(MTHREADS) k_momentum(
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi,
+    T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
+    T *__restrict__ ruo, T *__restrict__ rvo, T *__restrict__ rwo,
+    const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
+    Consts c, double ALPHA, double BETA, int kz)
+{
+    k_momentum_body<PHI, T>(xcd_block(), u, v, w, phi, us, vs, ws, ruo, rvo, rwo, sinf, cosf, sc, c, ALPHA, BETA, kz);
+}
+
+
 // ---------------------------------------------------------------------------
 // The same pass with two x-points per thread, used for FP32 state (in FP64
 // the pair needs 220 VGPRs and 95 KB of LDS: one wave per SIMD): 64 x 8 points
@@ -576,7 +597,7 @@ __device__ __forceinline__ void st2nt(T *p, T a, T b)
 }
 
 template <int PHI, typename T>
-__global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
+__device__ __forceinline__ void k_momentum_x2_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ phi, T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
     T *__restrict__ ruo, T *__restrict__ rvo, T *__restrict__ rwo,
@@ -590,7 +611,6 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
     __shared__ T FL[PHI ? 2 : 1][3][PHI ? PFY * PFX : 1];
     const int n = c.n;
     const int tid = threadIdx.x;
-    const uint3 blk = xcd_block();
     const int i0 = blk.x * (2 * PTX), j0 = blk.y * PTY, k0 = blk.z * kz;
     const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     MomK<T> q;
@@ -746,6 +766,18 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
     }
 }
 
+template <int PHI, typename T>
+__global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
+    const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+    const T *__restrict__ phi, T *__restrict__ us, T *__restrict__ vs, T *__restrict__ ws,
+    T *__restrict__ ruo, T *__restrict__ rvo, T *__restrict__ rwo,
+    const double *__restrict__ sinf, const double *__restrict__ cosf, const Scalars *__restrict__ sc,
+    Consts c, double ALPHA, double BETA, int kz)
+{
+    k_momentum_x2_body<PHI, T>(xcd_block(), u, v, w, phi, us, vs, ws, ruo, rvo, rwo, sinf, cosf, sc, c, ALPHA, BETA, kz);
+}
+
+
 // ---------------------------------------------------------------------------
 // FFT machinery: mixed-radix Stockham autosort in LDS (Govindaraju et al.,
 // SC'08), unnormalised.  A block holds `nlines` lines of length n; element p of
@@ -816,8 +848,7 @@ __device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, cons
 // Poisson RHS, then one complex FFT per pair of x-lines (j, j+1) whose
 // spectrum is split into the two half spectra (main.f90:715-750).
 template <typename T>
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
-    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+__device__ __forceinline__ void k_xfwd_body(uint3 blk, const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
     Consts c, FftPlan pl, int ppb, int ld)
 {
@@ -825,7 +856,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
     extern __shared__ double2 lds[];
     const int n = c.n, half = n / 2, nh = n / 2 + 1;
     const int lstride = n + 1;
-    const int pair0 = blockIdx.x * ppb;
+    const int pair0 = blk.x * ppb;
     const double cdiv = c.rho * c.dxi / c.dt;
     for (int e = threadIdx.x; e < ppb * n; e += blockDim.x) {
         int pl_ = e / n, i = e - pl_ * n;
@@ -861,9 +892,19 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd(
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+    double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
+    Consts c, FftPlan pl, int ppb, int ld)
+{
+    k_xfwd_body<T>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, spec, tw, sc, c, pl, ppb, ld);
+}
+
+
 // Column pass along y (forward or inverse), in place.  Block = (kx tile, k).
 template <int SG>
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__ spec,
+__device__ __forceinline__ void k_ypass_body(uint3 blk, double2 *__restrict__ spec,
                                                            const double2 *__restrict__ tw,
                                                            const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                            int tkx)
@@ -871,7 +912,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
     const int n = pl.n, nh = n / 2 + 1;
-    const int q0 = blockIdx.x * tkx, k = blockIdx.y;
+    const int q0 = blk.x * tkx, k = blk.y;
     const int nq = min(tkx, nh - q0);
     double2 *base = spec + (size_t)k * n * ld + q0;
     for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
@@ -886,9 +927,19 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__
     }
 }
 
+template <int SG>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__ spec,
+                                                           const double2 *__restrict__ tw,
+                                                           const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                           int tkx)
+{
+    k_ypass_body<SG>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, sc, pl, ld, tkx);
+}
+
+
 // Column pass along z: forward FFT, division by -k^2 and N^3 with the mean
 // mode set to zero (A19, main.f90:765-798), inverse FFT.  Block = (kx tile, j).
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restrict__ spec,
+__device__ __forceinline__ void k_zpoisson_body(uint3 blk, double2 *__restrict__ spec,
                                                               const double2 *__restrict__ tw,
                                                               const double *__restrict__ ktab,
                                                               const Scalars *__restrict__ sc, FftPlan pl, int ld,
@@ -897,7 +948,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restric
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
     const int n = pl.n, nh = n / 2 + 1;
-    const int q0 = blockIdx.x * tkx, j = blockIdx.y;
+    const int q0 = blk.x * tkx, j = blk.y;
     const int nq = min(tkx, nh - q0);
     const size_t rs = (size_t)n * ld;
     double2 *base = spec + (size_t)j * ld + q0;
@@ -929,9 +980,19 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restric
     }
 }
 
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restrict__ spec,
+                                                              const double2 *__restrict__ tw,
+                                                              const double *__restrict__ ktab,
+                                                              const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                              int tkx)
+{
+    k_zpoisson_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+}
+
+
 // Inverse x pass: Hermitian extension of the half spectra of lines j and j+1,
 // one complex inverse FFT per pair, p(.,j) = Re, p(.,j+1) = Im (A20-A21).
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restrict__ spec,
+__device__ __forceinline__ void k_xinv_body(uint3 blk, const double2 *__restrict__ spec,
                                                           double *__restrict__ p, const double2 *__restrict__ tw,
                                                           const Scalars *__restrict__ sc, FftPlan pl, int ppb,
                                                           int ld)
@@ -940,7 +1001,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restr
     extern __shared__ double2 lds[];
     const int n = pl.n, half = n / 2;
     const int lstride = n + 1;
-    const int pair0 = blockIdx.x * ppb;
+    const int pair0 = blk.x * ppb;
     for (int e = threadIdx.x; e < ppb * n; e += blockDim.x) {
         int pl_ = e / n, q = e - pl_ * n;
         int pair = pair0 + pl_;
@@ -971,6 +1032,15 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restr
         p[IDX(i, j + 1, k)] = z.y;
     }
 }
+
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv(const double2 *__restrict__ spec,
+                                                          double *__restrict__ p, const double2 *__restrict__ tw,
+                                                          const Scalars *__restrict__ sc, FftPlan pl, int ppb,
+                                                          int ld)
+{
+    k_xinv_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
+}
+
 
 // ---------------------------------------------------------------------------
 // Register-resident variant of the same Stockham FFT, for plans whose radices
@@ -1067,7 +1137,7 @@ __device__ __forceinline__ void nt_store2(double2 *dst, double re, double im)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
+__device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
     Consts c, FftPlan pl, int ppb, int ld)
@@ -1077,7 +1147,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
     const int n = c.n, half = n / 2, nh = n / 2 + 1, LM = n / 8;
     const int lstride = rline_stride(n);
     const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
-    const int pair = blockIdx.x * ppb + pl_;
+    const int pair = blk.x * ppb + pl_;
     const int k = pair / half, j = 2 * (pair - k * half);
     const int kp = k + 1 == n ? 0 : k + 1;
     const double cdiv = c.rho * c.dxi / c.dt;
@@ -1106,7 +1176,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
     __syncthreads();
     for (int e = t; e < ppb * nh; e += blockDim.x) {
         int pq = e / nh, q = e - pq * nh;
-        int pr = blockIdx.x * ppb + pq;
+        int pr = blk.x * ppb + pq;
         int kk = pr / half, jj = 2 * (pr - kk * half);
         int mq = q == 0 ? 0 : n - q;
         double2 zq = lds[rlds<true>(pq, q, lstride, 1)], zm = lds[rlds<true>(pq, mq, lstride, 1)];
@@ -1116,8 +1186,18 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+    double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
+    Consts c, FftPlan pl, int ppb, int ld)
+{
+    k_xfwd_r_body<T>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, spec, tw, sc, c, pl, ppb, ld);
+}
+
+
 template <int SG>
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict__ spec,
+__device__ __forceinline__ void k_ypass_r_body(uint3 blk, double2 *__restrict__ spec,
                                                              const double2 *__restrict__ tw,
                                                              const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                              int tkx)
@@ -1125,7 +1205,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
     const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
-    const int q0 = blockIdx.x * tkx, k = blockIdx.y;
+    const int q0 = blk.x * tkx, k = blk.y;
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const bool ok = q0 + line < nh;
     double2 *base = spec + (size_t)k * n * ld + q0 + line;
@@ -1139,7 +1219,17 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict
     }
 }
 
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restrict__ spec,
+template <int SG>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict__ spec,
+                                                             const double2 *__restrict__ tw,
+                                                             const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                             int tkx)
+{
+    k_ypass_r_body<SG>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, sc, pl, ld, tkx);
+}
+
+
+__device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict__ spec,
                                                                 const double2 *__restrict__ tw,
                                                                 const double *__restrict__ ktab,
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
@@ -1148,7 +1238,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restr
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
     const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
-    const int q0 = blockIdx.x * tkx, j = blockIdx.y;
+    const int q0 = blk.x * tkx, j = blk.y;
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const int q = q0 + line;
     const bool ok = q < nh;
@@ -1177,7 +1267,17 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restr
     }
 }
 
-__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__restrict__ spec,
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restrict__ spec,
+                                                                const double2 *__restrict__ tw,
+                                                                const double *__restrict__ ktab,
+                                                                const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                                int tkx)
+{
+    k_zpoisson_r_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+}
+
+
+__device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restrict__ spec,
                                                             double *__restrict__ p, const double2 *__restrict__ tw,
                                                             const Scalars *__restrict__ sc, FftPlan pl, int ppb,
                                                             int ld)
@@ -1187,7 +1287,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__res
     const int n = pl.n, half = n / 2, LM = n / 8;
     const int lstride = rline_stride(n);
     const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
-    const int pair = blockIdx.x * ppb + pl_;
+    const int pair = blk.x * ppb + pl_;
     const int k = pair / half, j = 2 * (pair - k * half);
     const double2 *s1 = spec + ((size_t)k * n + j) * ld;
     const double2 *s2 = spec + ((size_t)k * n + j + 1) * ld;
@@ -1216,6 +1316,15 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__res
         __builtin_nontemporal_store(v[m].y, &p[IDX(i, j + 1, k)]);
     }
 }
+
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__restrict__ spec,
+                                                            double *__restrict__ p, const double2 *__restrict__ tw,
+                                                            const Scalars *__restrict__ sc, FftPlan pl, int ppb,
+                                                            int ld)
+{
+    k_xinv_r_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
+}
+
 
 // ---------------------------------------------------------------------------
 // A22 projection (main.f90:873-887) with the per-x-line partial sums and
@@ -1254,7 +1363,7 @@ __device__ __forceinline__ double wave_max(double x)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BLK) k_project(
+__device__ __forceinline__ void k_project_body(uint3 blk,
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     const double *__restrict__ p, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
     double *__restrict__ lsum, double *__restrict__ lmax, const Scalars *__restrict__ sc, Consts c)
@@ -1262,7 +1371,7 @@ __global__ void __launch_bounds__(BLK) k_project(
     if (halted(sc)) return;
     const int n = c.n;
     const int lane = threadIdx.x & 63;
-    const size_t line = ((size_t)blockIdx.x * BLK + threadIdx.x) >> 6;
+    const size_t line = ((size_t)blk.x * BLK + threadIdx.x) >> 6;
     if (line >= (size_t)n * n) return;
     const int j = (int)(line % n), k = (int)(line / n);
     const int jm = j == 0 ? n - 1 : j - 1, km = k == 0 ? n - 1 : k - 1;
@@ -1298,14 +1407,24 @@ __global__ void __launch_bounds__(BLK) k_project(
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_project(
+    const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+    const double *__restrict__ p, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+    double *__restrict__ lsum, double *__restrict__ lmax, const Scalars *__restrict__ sc, Consts c)
+{
+    k_project_body<T>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, p, u, v, w, lsum, lmax, sc, c);
+}
+
+
 // Per z-plane: lane-64 sum over the n line sums of the plane (y direction).
 // One wavefront per (plane, component).
-__global__ void __launch_bounds__(64) k_reduce_planes(const double *__restrict__ lsum, const double *__restrict__ lmax,
+__device__ __forceinline__ void k_reduce_planes_body(uint3 blk, const double *__restrict__ lsum, const double *__restrict__ lmax,
                                                       double *__restrict__ psum, double *__restrict__ pmax,
                                                       const Scalars *__restrict__ sc, int n)
 {
     if (halted(sc)) return;
-    const int k = blockIdx.x, comp = blockIdx.y, lane = threadIdx.x;
+    const int k = blk.x, comp = blk.y, lane = threadIdx.x;
     const size_t nn = (size_t)n * n;
     const double *ls = lsum + comp * nn + (size_t)k * n;
     const double *lm = lmax + comp * nn + (size_t)k * n;
@@ -1322,8 +1441,16 @@ __global__ void __launch_bounds__(64) k_reduce_planes(const double *__restrict__
     }
 }
 
+__global__ void __launch_bounds__(64) k_reduce_planes(const double *__restrict__ lsum, const double *__restrict__ lmax,
+                                                      double *__restrict__ psum, double *__restrict__ pmax,
+                                                      const Scalars *__restrict__ sc, int n)
+{
+    k_reduce_planes_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), lsum, lmax, psum, pmax, sc, n);
+}
+
+
 // Means (main.f90:906-913), signed umax (937-941), CFL (942-945), halt flag.
-__global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum, const double *__restrict__ pmax,
+__device__ __forceinline__ void k_finalize_body(uint3 blk, const double *__restrict__ psum, const double *__restrict__ pmax,
                                                  Scalars *__restrict__ sc, Consts c, EnsSlot *slot)
 {
     if (sc->halted) return;
@@ -1368,6 +1495,137 @@ __global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum
         }
     }
 }
+
+__global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum, const double *__restrict__ pmax,
+                                                 Scalars *__restrict__ sc, Consts c, EnsSlot *slot)
+{
+    k_finalize_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), psum, pmax, sc, c, slot);
+}
+
+// ---------------------------------------------------------------------------
+// Gang launches: one pass of one step for up to GANG_MAX boxes of the same n,
+// precision and phiflag in a single launch.  The box is one more grid
+// dimension (folded into z for the marching kernels, so that the XCD-aware
+// block order hands each XCD whole boxes); every block loads its box's
+// buffers and constants from the descriptor array g and runs the one-box code
+// unchanged, so a box gives the same bits in a gang as alone.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned gang_fold(uint3 &b, int nbox)
+{
+    const unsigned gz = gridDim.z / (unsigned)nbox, box = b.z / gz;
+    b.z -= box * gz;
+    return box;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) kg_acdi(const GangBox *__restrict__ g, int nbox,
+                                                                           int kz)
+{
+    uint3 b = xcd_block();
+    const GangBox &B = g[gang_fold(b, nbox)];
+    k_acdi_body<T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phi, (T *)B.f.phin,
+                   B.f.sc, B.f.mtab, B.c, kz);
+}
+
+// euler: bit b set when box b takes its first (Euler) step, main.f90:323-324.
+// The CSF force uses phi^{n+1} (main.f90:604-658): the buffer the phase pass
+// of this step has just written, phin in the step's descriptor.
+// At most 128 VGPRs (4 waves per SIMD), as the one-box kernel compiles to.
+template <int PHI, typename T>
+__global__ void __launch_bounds__(MTHREADS, 4) kg_momentum(const GangBox *__restrict__ g, int nbox, unsigned euler,
+                                                        int kz)
+{
+    uint3 b = xcd_block();
+    const unsigned box = gang_fold(b, nbox);
+    const GangBox &B = g[box];
+    const bool eu = (euler >> box) & 1u;
+    k_momentum_body<PHI, T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin,
+                            (T *)B.f.us, (T *)B.f.vs, (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo,
+                            B.f.sinf, B.f.cosf, B.f.sc, B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
+}
+
+template <int PHI, typename T>
+__global__ void __launch_bounds__(PTHREADS) kg_momentum_x2(const GangBox *__restrict__ g, int nbox, unsigned euler,
+                                                           int kz)
+{
+    uint3 b = xcd_block();
+    const unsigned box = gang_fold(b, nbox);
+    const GangBox &B = g[box];
+    const bool eu = (euler >> box) & 1u;
+    k_momentum_x2_body<PHI, T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin,
+                               (T *)B.f.us, (T *)B.f.vs, (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo,
+                               B.f.sinf, B.f.cosf, B.f.sc, B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
+}
+
+// FFT passes: box = the first grid dimension the one-box launch leaves unused.
+template <typename T, bool REGS>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) kg_xfwd(const GangBox *__restrict__ g, FftPlan pl, int ppb, int ld)
+{
+    const GangBox &B = g[blockIdx.y];
+    const uint3 b = make_uint3(blockIdx.x, 0, 0);
+    if constexpr (REGS)
+        k_xfwd_r_body<T>(b, (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.spec, B.f.tw, B.f.sc, B.c,
+                         pl, ppb, ld);
+    else
+        k_xfwd_body<T>(b, (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.spec, B.f.tw, B.f.sc, B.c, pl,
+                       ppb, ld);
+}
+
+template <int SG, bool REGS>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) kg_ypass(const GangBox *__restrict__ g, FftPlan pl, int ld, int tkx)
+{
+    const GangBox &B = g[blockIdx.z];
+    const uint3 b = make_uint3(blockIdx.x, blockIdx.y, 0);
+    if constexpr (REGS)
+        k_ypass_r_body<SG>(b, B.f.spec, B.f.tw, B.f.sc, pl, ld, tkx);
+    else
+        k_ypass_body<SG>(b, B.f.spec, B.f.tw, B.f.sc, pl, ld, tkx);
+}
+
+template <bool REGS>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) kg_zpoisson(const GangBox *__restrict__ g, FftPlan pl, int ld,
+                                                               int tkx)
+{
+    const GangBox &B = g[blockIdx.z];
+    const uint3 b = make_uint3(blockIdx.x, blockIdx.y, 0);
+    if constexpr (REGS)
+        k_zpoisson_r_body(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
+    else
+        k_zpoisson_body(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
+}
+
+template <bool REGS>
+__global__ void __launch_bounds__(FFT_MAX_THREADS) kg_xinv(const GangBox *__restrict__ g, FftPlan pl, int ppb, int ld)
+{
+    const GangBox &B = g[blockIdx.y];
+    const uint3 b = make_uint3(blockIdx.x, 0, 0);
+    if constexpr (REGS)
+        k_xinv_r_body(b, B.f.spec, B.f.p, B.f.tw, B.f.sc, pl, ppb, ld);
+    else
+        k_xinv_body(b, B.f.spec, B.f.p, B.f.tw, B.f.sc, pl, ppb, ld);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLK) kg_project(const GangBox *__restrict__ g)
+{
+    const GangBox &B = g[blockIdx.y];
+    k_project_body<T>(make_uint3(blockIdx.x, 0, 0), (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.p,
+                      (T *)B.f.u, (T *)B.f.v, (T *)B.f.w, B.f.lsum, B.f.lmax, B.f.sc, B.c);
+}
+
+__global__ void __launch_bounds__(64) kg_reduce_planes(const GangBox *__restrict__ g)
+{
+    const GangBox &B = g[blockIdx.z];
+    k_reduce_planes_body(make_uint3(blockIdx.x, blockIdx.y, 0), B.f.lsum, B.f.lmax, B.f.psum, B.f.pmax, B.f.sc,
+                         B.c.n);
+}
+
+__global__ void __launch_bounds__(64) kg_finalize(const GangBox *__restrict__ g)
+{
+    const GangBox &B = g[blockIdx.x];
+    k_finalize_body(make_uint3(0, 0, 0), B.f.psum, B.f.pmax, B.f.sc, B.c, B.f.slot);
+}
+
 
 // ---------------------------------------------------------------------------
 // On-device diagnostics: kinetic energy 0.5<u.u>, the dissipation
@@ -1884,6 +2142,84 @@ void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, f.psum, f.pmax, f.sc, c, f.slot);
     li->name = "finalize"; li->bytes = 6.0 * 8.0 * c.n;
+}
+
+// One step of a gang (device descriptors g[0..nbox), all of c's n, precision
+// and phiflag), with the same launch shapes as the one-box passes above.
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &c, const FftPlan &pl, int ld,
+                      const Tuning &tu, hipStream_t s)
+{
+    const int n = c.n, nh = n / 2 + 1, pairs = n * n / 2;
+    if (c.phiflag) {
+        const int kz = tu.v[KNOB_KZ_ACDI] ? tu.v[KNOB_KZ_ACDI] : march_kz(n, ATX, ATY);
+        const dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, (n / kz) * nbox);
+        if (c.mixed)
+            hipLaunchKernelGGL(kg_acdi<float>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
+        else
+            hipLaunchKernelGGL(kg_acdi<double>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
+    }
+    {
+        const int kz = tu.v[KNOB_KZ_MOM] ? tu.v[KNOB_KZ_MOM] : march_kz(n, MTX, MTY);
+        if (c.mixed) {
+            const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, (n / kz) * nbox);
+            if (c.phiflag)
+                hipLaunchKernelGGL((kg_momentum_x2<1, float>), g2, dim3(PTHREADS), 0, s, g, nbox, euler, kz);
+            else
+                hipLaunchKernelGGL((kg_momentum_x2<0, float>), g2, dim3(PTHREADS), 0, s, g, nbox, euler, kz);
+        } else {
+            const dim3 grid((n + MTX - 1) / MTX, (n + MTY - 1) / MTY, (n / kz) * nbox);
+            if (c.phiflag)
+                hipLaunchKernelGGL((kg_momentum<1, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, kz);
+            else
+                hipLaunchKernelGGL((kg_momentum<0, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, kz);
+        }
+    }
+    if (plan_regs(pl)) {
+        const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl);
+        const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl);
+        const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl);
+        const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl);
+        const size_t shx = sizeof(double2) * (size_t)xp * rline_stride(n);
+        if (c.mixed)
+            hipLaunchKernelGGL((kg_xfwd<float, true>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g, pl, xp,
+                               ld);
+        else
+            hipLaunchKernelGGL((kg_xfwd<double, true>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g, pl, xp,
+                               ld);
+        const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
+        const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
+        hipLaunchKernelGGL((kg_ypass<-1, true>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL(kg_zpoisson<true>, gz, dim3(tz * (n / 8)), shz, s, g, pl, ld, tz);
+        hipLaunchKernelGGL((kg_ypass<1, true>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL(kg_xinv<true>, dim3(pairs / xip, nbox), dim3(xip * (n / 8)),
+                           sizeof(double2) * (size_t)xip * rline_stride(n), s, g, pl, xip, ld);
+    } else {
+        const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : pairs_per_block(pl);
+        const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : pairs_per_block(pl);
+        const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : col_tile(pl);
+        const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
+        const size_t shx = sizeof(double2) * (size_t)xp * (n + 1);
+        if (c.mixed)
+            hipLaunchKernelGGL((kg_xfwd<float, false>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
+                               pl, xp, ld);
+        else
+            hipLaunchKernelGGL((kg_xfwd<double, false>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
+                               pl, xp, ld);
+        const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
+        const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
+        hipLaunchKernelGGL((kg_ypass<-1, false>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL(kg_zpoisson<false>, gz, dim3(fft_threads(pl, tz)), shz, s, g, pl, ld, tz);
+        hipLaunchKernelGGL((kg_ypass<1, false>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL(kg_xinv<false>, dim3(pairs / xip, nbox), dim3(fft_threads(pl, xip)),
+                           sizeof(double2) * (size_t)xip * (n + 1), s, g, pl, xip, ld);
+    }
+    const dim3 gp((unsigned)nblocks((size_t)n * n * 64, BLK), nbox);
+    if (c.mixed)
+        hipLaunchKernelGGL(kg_project<float>, gp, dim3(BLK), 0, s, g);
+    else
+        hipLaunchKernelGGL(kg_project<double>, gp, dim3(BLK), 0, s, g);
+    hipLaunchKernelGGL(kg_reduce_planes, dim3(n, 3, nbox), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(kg_finalize, dim3(nbox), dim3(64), 0, s, g);
 }
 
 }  // namespace mhit

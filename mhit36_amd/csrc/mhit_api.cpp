@@ -109,6 +109,7 @@ struct mhit_ctx {
     size_t event_next = 0;
     std::vector<PendingTiming> pending;
     std::map<std::string, KStat> stats;
+    GangBox *gang = nullptr;     // device descriptors of mhit_step_batch (2 x GANG_MAX)
 };
 
 #define HIPCHK(expr)                                                     \
@@ -711,6 +712,102 @@ extern "C" int mhit_step(mhit_ctx *c, int nsteps, mhit_diag *d)
         d->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return s1.halted ? s1.status : MHIT_OK;
+}
+
+// Gang stepping: one launch per pass for several boxes of one class.  The
+// descriptors of both phi orientations (as now, and swapped) go to the lead
+// context's device buffer once per call; step s reads orientation s % 2.
+static_assert(GANG_MAX == MHIT_BATCH_MAX, "gang size");
+
+extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_diag *d)
+{
+    if (!ctx || nbox < 1 || nbox > GANG_MAX || nsteps < 0) return MHIT_ERR_ARG;
+    for (int b = 0; b < nbox; b++) {
+        if (!ctx[b]) return MHIT_ERR_ARG;
+        const Consts &k = ctx[b]->kc, &k0 = ctx[0]->kc;
+        if (k.n != k0.n || k.mixed != k0.mixed || k.phiflag != k0.phiflag || ctx[b]->device != ctx[0]->device)
+            return MHIT_ERR_ARG;
+        for (int q = 0; q < b; q++)
+            if (ctx[q] == ctx[b]) return MHIT_ERR_ARG;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    mhit_ctx *lead = ctx[0];
+    if (!lead->gang) {
+        void *g = nullptr;
+        int rc = dalloc(lead, &g, sizeof(GangBox) * 2 * GANG_MAX);
+        if (rc) return rc;
+        lead->gang = (GangBox *)g;
+    }
+    std::vector<mhit_ctx *> run;                  // boxes that step
+    std::vector<Scalars> s0(nbox);
+    std::vector<GangBox> desc(2 * GANG_MAX);
+    for (int b = 0; b < nbox; b++) {
+        mhit_ctx *c = ctx[b];
+        HIPCHK(hipStreamSynchronize(c->stream));  // its own earlier work
+        if (c->halted) continue;
+        int rc = read_scalars(c, &s0[b]);
+        if (rc) return rc;
+        const int m = (int)run.size();
+        desc[m].f = c->f;
+        desc[m].c = c->kc;
+        desc[GANG_MAX + m] = desc[m];
+        std::swap(desc[GANG_MAX + m].f.phi, desc[GANG_MAX + m].f.phin);
+        run.push_back(c);
+    }
+    const int m = (int)run.size();
+    std::vector<int> since0(m);
+    if (m > 0) {
+        hipStream_t s = lead->stream;
+        HIPCHK(hipMemcpyAsync(lead->gang, desc.data(), sizeof(GangBox) * GANG_MAX, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(lead->gang + GANG_MAX, desc.data() + GANG_MAX, sizeof(GangBox) * GANG_MAX,
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        unsigned euler = 0;
+        for (int q = 0; q < m; q++) {
+            since0[q] = run[q]->nsteps_since_init;
+            if (since0[q] == 0) euler |= 1u << q;
+        }
+        for (int st = 0; st < nsteps; st++) {
+            const GangBox *g = lead->gang + (run[0]->kc.phiflag && (st & 1) ? GANG_MAX : 0);
+            launch_gang_step(g, m, st == 0 ? euler : 0u, lead->kc, lead->plan, lead->f.ld, lead->tune, s);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    int first = MHIT_OK;
+    for (int b = 0, q = 0; b < nbox; b++) {
+        mhit_ctx *c = ctx[b];
+        Scalars s1;
+        int rc = read_scalars(c, &s1);
+        if (rc) return rc;
+        int done = 0;
+        if (!c->halted) {
+            done = s1.steps - s0[b].steps;
+            if (c->kc.phiflag && done % 2) std::swap(c->f.phi, c->f.phin);
+            c->nsteps_since_init = since0[q] + done;
+            c->t += done;
+            if (s1.halted) {
+                c->halted = 1;
+                c->halt_status = s1.status;
+            }
+            q++;
+        }
+        const int status = c->halted ? c->halt_status : MHIT_OK;
+        if (status && !first) first = status;
+        if (d) {
+            d[b].steps_done = done;
+            d[b].t = c->t;
+            d[b].status = status;
+            d[b].gumax = s1.gumax;
+            d[b].cou = s1.cou;
+            d[b].umean = s1.umean;
+            d[b].vmean = s1.vmean;
+            d[b].wmean = s1.wmean;
+            d[b].ms = ms;
+        }
+    }
+    return first;
 }
 
 // ---------------------------------------------------------------------------

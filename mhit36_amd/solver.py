@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import Params, Consts, Diag, KStat, Stats, Tuning, MHIT_TUNE_FORCE, check, MhitError
+from ._lib import Params, Consts, Diag, KStat, Stats, Tuning, MHIT_TUNE_FORCE, MHIT_BATCH_MAX, check, MhitError
 
 FIELD_NAMES = {1: "u", 2: "v", 3: "w", 4: "p", 5: "phi"}   # readwrite.f90:21-25
 
@@ -214,3 +214,22 @@ class Solver:
         b = ctypes.c_size_t(0)
         check(self._lib.mhit_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
         return b.value
+
+
+def step_batch(solvers, nsteps=1, raise_on_stop=True):
+    """Advance several boxes of one nx, precision and phiflag together: one
+    launch per pass for all of them (mhit_step_batch).  Each box ends
+    bit-identical to Solver.step on it alone.  Returns one Diag per box."""
+    solvers = list(solvers)
+    if not 1 <= len(solvers) <= MHIT_BATCH_MAX:
+        raise ValueError(f"1..{MHIT_BATCH_MAX} boxes per batch, got {len(solvers)}")
+    lib = _lib.load()
+    hs = (ctypes.c_void_p * len(solvers))(*[s._h.value for s in solvers])
+    ds = (Diag * len(solvers))()
+    rc = lib.mhit_step_batch(hs, len(solvers), int(nsteps), ds)
+    out = list(ds)
+    for s, d in zip(solvers, out):
+        s.last = d
+    if rc != _lib.MHIT_OK and (raise_on_stop or rc not in (_lib.MHIT_ERR_CFL, _lib.MHIT_ERR_NAN)):
+        raise MhitError(rc, "mhit_step_batch")
+    return out
