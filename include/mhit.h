@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MHIT_ABI_VERSION 2
+#define MHIT_ABI_VERSION 3
 
 enum {
     MHIT_OK = 0,
@@ -237,6 +237,10 @@ typedef struct mhit_ens_config {
     int inflight;             /* steps queued ahead per box (0: 2)                       */
     int use_graphs;           /* 1: replay each box's step as a captured hipGraph        */
     int autotune;             /* 1: autotune each new (nx, precision, phiflag) at admission */
+    int gang_size;            /* boxes of one class stepped by one launch per pass (0: 8; 1: off;
+                                 at most MHIT_BATCH_MAX), as mhit_step_batch                 */
+    int gang_max_nx;          /* boxes with nx <= this step in gangs (0: 128); larger boxes
+                                 fill the GPU alone and replay their own step graphs          */
 } mhit_ens_config;
 
 typedef struct mhit_ens_result {

@@ -1366,6 +1366,9 @@ struct EnsReq {
     hipGraphExec_t g_euler = nullptr, g_ab2[2] = {nullptr, nullptr};
     int launched = 0, done = 0;
     EnsSlot *slot = nullptr;             // host-mapped progress record
+    bool ganged = false;                 // stepped by gang launches (small nx)
+    struct EnsGang *gang = nullptr;      // the gang it joined at admission
+    bool in_gang = false;                // still a member (takes further steps)
     int status = MHIT_OK;
     int job_rc = MHIT_OK;                // result of the last I/O-thread job
     double gumax = 0, cou = 0;
@@ -1380,6 +1383,24 @@ constexpr int ENS_IO_THREADS = 4;
 struct Job {
     JobKind kind;
     EnsReq *r;
+};
+
+// Boxes of one (nx, precision, phiflag) stepped together: one launch per pass
+// for all members (continuous batching: boxes join and leave between steps).
+// Each change of membership uploads a fresh descriptor set into the next slot
+// of a ring in device memory; a slot is reused only after the event recorded
+// behind its last launch has completed.
+constexpr int GANG_RING = 32;
+struct EnsGang {
+    int nx = 0, mixed = 0, phiflag = 0;
+    hipStream_t stream = nullptr;
+    std::vector<EnsReq *> members;       // boxes that still take steps, descriptor order
+    bool dirty = true;                   // members changed since the last upload
+    GangBox *dev = nullptr;              // GANG_RING x (2 x GANG_MAX) descriptors
+    GangBox *host = nullptr;             // pinned staging, same layout
+    hipEvent_t ev[GANG_RING] = {};
+    bool ev_live[GANG_RING] = {};
+    int slot = -1, parity = 0;
 };
 
 }  // namespace
@@ -1400,6 +1421,7 @@ struct mhit_ens {
     std::vector<EnsSlot *> free_slots;
     std::vector<EnsSlot *> all_slots;
     double clk_khz = 0.0;                // device wall clock rate
+    std::vector<std::unique_ptr<EnsGang>> gangs;
     // I/O threads
     std::vector<std::thread> io;
     std::mutex mu;
@@ -1473,7 +1495,7 @@ static int io_admit(mhit_ens *e, EnsReq *r)
     EnsSlot *dslot = nullptr;
     if (rc == MHIT_OK && hipHostGetDevicePointer((void **)&dslot, r->slot, 0) != hipSuccess) rc = MHIT_ERR_HIP;
     c->f.slot = dslot;
-    if (rc == MHIT_OK && e->cfg.use_graphs) {
+    if (rc == MHIT_OK && e->cfg.use_graphs && !r->ganged) {
         // Euler first step from parity 0, then AB2 steps at parity 1, 0, 1, ...
         rc = capture_step(c, &r->g_euler);
         c->nsteps_since_init = 1;
@@ -1566,6 +1588,126 @@ static void record_latency(mhit_ens *e, EnsReq *r)
     }
 }
 
+// A gang of r's class with room, or a new one.
+static int join_gang(mhit_ens *e, EnsReq *r)
+{
+    const Consts &k = r->ctx->kc;
+    EnsGang *gg = nullptr;
+    for (auto &g : e->gangs)
+        if ((int)g->members.size() < e->cfg.gang_size &&
+            (g->members.empty() || (g->nx == k.n && g->mixed == k.mixed && g->phiflag == k.phiflag))) {
+            gg = g.get();
+            if (!g->members.empty()) break;      // prefer a running gang of the class
+        }
+    if (!gg) {
+        auto g = std::make_unique<EnsGang>();
+        const size_t bytes = sizeof(GangBox) * 2 * GANG_MAX * GANG_RING;
+        if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) return MHIT_ERR_HIP;
+        if (hipMalloc((void **)&g->dev, bytes) != hipSuccess ||
+            hipHostMalloc((void **)&g->host, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            if (g->dev) (void)hipFree(g->dev);
+            (void)hipStreamDestroy(g->stream);
+            return MHIT_ERR_OOM;
+        }
+        for (int q = 0; q < GANG_RING; q++)
+            if (hipEventCreateWithFlags(&g->ev[q], hipEventDisableTiming) != hipSuccess) return MHIT_ERR_HIP;
+        gg = g.get();
+        e->gangs.push_back(std::move(g));
+    }
+    gg->nx = k.n;
+    gg->mixed = k.mixed;
+    gg->phiflag = k.phiflag;
+    gg->members.push_back(r);
+    gg->dirty = true;
+    r->gang = gg;
+    r->in_gang = true;
+    return MHIT_OK;
+}
+
+static void leave_gang(EnsReq *r)
+{
+    EnsGang *g = r->gang;
+    if (!g || !r->in_gang) return;
+    g->members.erase(std::find(g->members.begin(), g->members.end(), r));
+    g->dirty = true;
+    r->in_gang = false;
+}
+
+// Descriptors of the current members into the next ring slot: set 0 with each
+// box's phi buffers as its next step reads them, set 1 swapped.
+static int gang_upload(EnsGang *g)
+{
+    if (g->slot >= 0) {
+        HIPCHK(hipEventRecord(g->ev[g->slot], g->stream));
+        g->ev_live[g->slot] = true;
+    }
+    const int s = (g->slot + 1) % GANG_RING;
+    if (g->ev_live[s]) HIPCHK(hipEventSynchronize(g->ev[s]));
+    GangBox *h = g->host + (size_t)s * 2 * GANG_MAX;
+    for (size_t b = 0; b < g->members.size(); b++) {
+        const EnsReq *r = g->members[b];
+        GangBox &d0 = h[b], &d1 = h[GANG_MAX + b];
+        d0.f = r->ctx->f;
+        d0.c = r->ctx->kc;
+        if (g->phiflag) {
+            const bool odd = r->launched % 2;
+            d0.f.phi = odd ? r->phin0 : r->phi0;
+            d0.f.phin = odd ? r->phi0 : r->phin0;
+        }
+        d1 = d0;
+        std::swap(d1.f.phi, d1.f.phin);
+    }
+    HIPCHK(hipMemcpyAsync(g->dev + (size_t)s * 2 * GANG_MAX, h, sizeof(GangBox) * 2 * GANG_MAX,
+                          hipMemcpyHostToDevice, g->stream));
+    g->slot = s;
+    g->parity = 0;
+    g->dirty = false;
+    return MHIT_OK;
+}
+
+// Queue gang steps while every member has fewer than `inflight` steps queued;
+// members that have all their steps queued, or have stopped, leave the gang.
+static int gang_progress(mhit_ens *e, EnsGang *g)
+{
+    for (;;) {
+        for (size_t b = 0; b < g->members.size();) {
+            EnsReq *r = g->members[b];
+            if (r->launched >= r->nsteps || r->slot->halted) leave_gang(r);
+            else b++;
+        }
+        if (g->members.empty()) return MHIT_OK;
+        int ahead = 0;
+        for (EnsReq *r : g->members) ahead = std::max(ahead, r->launched - r->slot->steps);
+        if (ahead >= e->cfg.inflight) return MHIT_OK;
+        if (g->dirty) {
+            int rc = gang_upload(g);
+            if (rc) return rc;
+        }
+        unsigned euler = 0;
+        for (size_t b = 0; b < g->members.size(); b++)
+            if (g->members[b]->launched == 0) euler |= 1u << b;
+        const mhit_ctx *c0 = g->members[0]->ctx;
+        launch_gang_step(g->dev + (size_t)g->slot * 2 * GANG_MAX + (g->phiflag && g->parity ? GANG_MAX : 0),
+                         (int)g->members.size(), euler, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
+        HIPCHK(hipGetLastError());
+        g->parity ^= 1;
+        for (EnsReq *r : g->members) r->launched++;
+    }
+}
+
+// Before a ganged box retires, its own stream waits for every gang launch
+// queued so far (a stopped box's descriptors may still be read by them).
+static int gang_fence(EnsGang *g, EnsReq *r)
+{
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, g->stream));
+    HIPCHK(hipStreamWaitEvent(r->ctx->stream, ev, 0));
+    HIPCHK(hipEventDestroy(ev));
+    return MHIT_OK;
+}
+
 extern "C" int mhit_ens_create(const mhit_ens_config *cfg, mhit_ens **out)
 {
     if (!out) return MHIT_ERR_ARG;
@@ -1578,6 +1720,9 @@ extern "C" int mhit_ens_create(const mhit_ens_config *cfg, mhit_ens **out)
     if (cfg) e->cfg = *cfg;
     if (e->cfg.max_active <= 0) e->cfg.max_active = 64;
     if (e->cfg.inflight <= 0) e->cfg.inflight = 2;
+    if (e->cfg.gang_size <= 0) e->cfg.gang_size = 8;
+    if (e->cfg.gang_size > GANG_MAX) e->cfg.gang_size = GANG_MAX;
+    if (e->cfg.gang_max_nx <= 0) e->cfg.gang_max_nx = 128;
     if (e->cfg.hbm_budget_bytes == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return MHIT_ERR_HIP; }
@@ -1614,6 +1759,7 @@ extern "C" int mhit_ens_submit(mhit_ens *e, const mhit_params *p, const double *
     r->p = *p;
     r->nsteps = nsteps;
     r->bytes = ctx_bytes(p);
+    r->ganged = e->cfg.gang_size > 1 && p->nx <= e->cfg.gang_max_nx;
     const size_t n3 = (size_t)p->nx * p->nx * p->nx;
     const double *src[4] = {u, v, w, phi};
     if (u && v && w)
@@ -1655,6 +1801,7 @@ extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
             continue;
         }
         e->admitting--;
+        if (r->job_rc == MHIT_OK && r->ganged) r->job_rc = join_gang(e, r);
         if (r->job_rc == MHIT_OK) {
             r->admit_ms = ens_now(e);
             e->active.push_back(r);
@@ -1704,7 +1851,7 @@ extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
         mhit_ctx *c = r->ctx;
         r->done = r->slot->steps;
         const bool halted = r->slot->halted != 0;
-        while (!halted && r->launched < r->nsteps && r->launched - r->done < e->cfg.inflight) {
+        while (!r->ganged && !halted && r->launched < r->nsteps && r->launched - r->done < e->cfg.inflight) {
             if (e->cfg.use_graphs) {
                 hipGraphExec_t g = r->launched == 0 ? r->g_euler
                                                     : r->g_ab2[c->kc.phiflag ? r->launched % 2 : 0];
@@ -1715,6 +1862,11 @@ extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
             r->launched++;
         }
         if (r->done >= r->nsteps || halted) {
+            if (r->ganged) {
+                leave_gang(r);
+                int rc = gang_fence(r->gang, r);
+                if (rc) return rc;
+            }
             record_latency(e, r);
             e->active.erase(e->active.begin() + a);
             e->retiring++;
@@ -1722,6 +1874,11 @@ extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
             continue;
         }
         a++;
+    }
+    // 4. gang steps of the small boxes
+    for (auto &g : e->gangs) {
+        int rc = gang_progress(e, g.get());
+        if (rc) return rc;
     }
     if (n_active) *n_active = (int)e->active.size() + e->admitting + e->retiring;
     if (n_queued) *n_queued = (int)e->queue.size();
@@ -1817,6 +1974,7 @@ extern "C" int mhit_ens_destroy(mhit_ens *e)
     }
     e->cv.notify_all();
     for (std::thread &t : e->io) t.join();   // they run the jobs already posted first
+    for (auto &g : e->gangs) (void)hipStreamSynchronize(g->stream);
     for (EnsReq *r : e->active) {
         if (!r->ctx) continue;
         (void)hipStreamSynchronize(r->ctx->stream);
@@ -1832,6 +1990,13 @@ extern "C" int mhit_ens_destroy(mhit_ens *e)
         }
     for (auto &kv : e->all)
         if (kv.second->stream) e->streams.push_back(kv.second->stream);
+    for (auto &g : e->gangs) {
+        for (hipEvent_t ev : g->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(g->stream);
+        (void)hipFree(g->dev);
+        (void)hipHostFree(g->host);
+    }
     for (EnsSlot *sl : e->all_slots) (void)hipHostFree(sl);
     for (hipStream_t st : e->streams) (void)hipStreamDestroy(st);
     if (e->arena.base) (void)hipFree(e->arena.base);

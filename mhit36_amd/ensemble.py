@@ -11,7 +11,8 @@ from .solver import default_params
 
 class EnsConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_active", ctypes.c_int),
-                ("inflight", ctypes.c_int), ("use_graphs", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("inflight", ctypes.c_int), ("use_graphs", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("gang_size", ctypes.c_int), ("gang_max_nx", ctypes.c_int)]
 
 
 class EnsResult(ctypes.Structure):
@@ -47,10 +48,14 @@ def _bind(lib):
 
 
 class Ensemble:
-    def __init__(self, hbm_budget_bytes=0, max_active=64, inflight=2, use_graphs=True, autotune=True):
+    def __init__(self, hbm_budget_bytes=0, max_active=64, inflight=2, use_graphs=True, autotune=True,
+                 gang_size=8, gang_max_nx=128):
+        """gang_size: boxes of one (nx, precision, phiflag) with nx <= gang_max_nx
+        stepped together by one launch per pass (1: every box on its own)."""
         self._lib = _lib.load()
         _bind(self._lib)
-        cfg = EnsConfig(hbm_budget_bytes, max_active, inflight, 1 if use_graphs else 0, 1 if autotune else 0)
+        cfg = EnsConfig(hbm_budget_bytes, max_active, inflight, 1 if use_graphs else 0, 1 if autotune else 0,
+                        gang_size, gang_max_nx)
         h = ctypes.c_void_p()
         check(self._lib.mhit_ens_create(ctypes.byref(cfg), ctypes.byref(h)), "mhit_ens_create")
         self._h = h

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_strerror():
     lib = _lib.load()
-    assert lib.mhit_abi_version() == 2
+    assert lib.mhit_abi_version() == 3
     for code in range(10):
         assert _lib.strerror(code)
     assert _lib.strerror(99) == "unknown error"

@@ -26,10 +26,10 @@ def alone(p, nsteps):
         return {f: s.get_field(f) for f in ((1, 2, 3, 4, 5) if p["phiflag"] else (1, 2, 3, 4))}
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_same_alone_and_in_mix(graphs):
+@pytest.mark.parametrize("graphs,gang", [(True, 1), (False, 1), (True, 8)])
+def test_same_alone_and_in_mix(graphs, gang):
     ref = [alone(p, n) for p, n in zip(BOXES, STEPS)]
-    with Ensemble(use_graphs=graphs, inflight=3) as ens:
+    with Ensemble(use_graphs=graphs, inflight=3, gang_size=gang) as ens:
         ids = [ens.submit(default_params(**p), nsteps=n) for p, n in zip(BOXES, STEPS)]
         ens.wait_idle()
         res = {r["id"]: r for r in ens.results()}
@@ -57,9 +57,10 @@ def test_budget_admission_serialises():
             assert np.array_equal(ens.get_field(rid, 1), ref[1])
 
 
-def test_halted_box_does_not_disturb_others():
+@pytest.mark.parametrize("bad_nx", [16, 32])      # 32: the stopping box shares the good box's gang
+def test_halted_box_does_not_disturb_others(bad_nx):
     good = dict(nx=32, dt=1e-3, mu=0.01, phiflag=0, precision=0)
-    bad = dict(nx=16, dt=3.0, mu=0.0, phiflag=0, precision=0)
+    bad = dict(nx=bad_nx, dt=3.0, mu=0.0, phiflag=0, precision=0)
     ref = alone(good, 10)
     with Ensemble() as ens:
         g = ens.submit(default_params(**good), nsteps=10)
@@ -121,3 +122,33 @@ def test_results_arrive_while_others_run():
             done[r["id"]] = r
         assert done[late]["admit_ms"] < done[lid]["finish_ms"]
         assert np.array_equal(ens.get_field(late, 1), ens.get_field(sid, 1))
+
+
+def test_gangs_take_and_release_boxes():
+    """Continuous batching inside gangs: twelve boxes of one class with
+    different step counts, submitted in three waves while earlier boxes run,
+    through gangs of four (members leave when done, later boxes join between
+    steps).  Every box equals the same box alone, and every step after each
+    box's first has a latency."""
+    _, dx = ic.grid(32)
+    ps, steps = [], []
+    for q in range(12):
+        p = dict(nx=32, dt=(0.04 + 0.005 * q) * dx, mu=0.004 + 0.002 * q, phiflag=1, radius=0.5,
+                 sigma=0.05, gamma_factor=1.2, precision=1, f1=0.5 * (q % 3))
+        ps.append(p)
+        steps.append(3 + (7 * q) % 11)
+    ref = [alone(p, n) for p, n in zip(ps, steps)]
+    with Ensemble(inflight=2, gang_size=4) as ens:
+        ids = []
+        for wave in range(3):
+            for q in range(4 * wave, 4 * wave + 4):
+                ids.append(ens.submit(default_params(**ps[q]), nsteps=steps[q]))
+            for _ in range(3):
+                ens.progress()
+        ens.wait_idle()
+        res = {r["id"]: r for r in ens.results()}
+        for q, rid in enumerate(ids):
+            assert res[rid]["status"] == 0 and res[rid]["steps_done"] == steps[q]
+            for f, a in ref[q].items():
+                assert np.array_equal(ens.get_field(rid, f), a), (q, f)
+        assert ens.latency()["steps"] == sum(steps) - len(steps)
