@@ -1869,11 +1869,12 @@ void launch_debug_math(int op, size_t count, const double *x, double *y, const d
 static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
 
 // Planes per block for the z-marching kernels: the largest of 32..2 dividing
-// n that still gives >= 2048 blocks, else the largest of 8..2 dividing n.
-static int march_kz(int n, int tx, int ty)
+// n that still gives >= 2048 blocks (over `boxes` boxes launched together),
+// else the largest of 8..2 dividing n.
+static int march_kz(int n, int tx, int ty, int boxes = 1)
 {
     for (int kz = 32; kz > 1; kz >>= 1)
-        if (n % kz == 0 && (n / kz) * ((n + tx - 1) / tx) * ((n + ty - 1) / ty) >= 2048) return kz;
+        if (n % kz == 0 && boxes * (n / kz) * ((n + tx - 1) / tx) * ((n + ty - 1) / ty) >= 2048) return kz;
     for (int kz = 8; kz > 1; kz >>= 1)
         if (n % kz == 0) return kz;
     return 1;
@@ -2150,8 +2151,11 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
                       const Tuning &tu, hipStream_t s)
 {
     const int n = c.n, nh = n / 2 + 1, pairs = n * n / 2;
+    // z planes per block from the gang's total block count (the boxes give the
+    // parallelism that thin slabs give one box); the FFT shapes are per line
+    // or column and keep the box's tuned values.
     if (c.phiflag) {
-        const int kz = tu.v[KNOB_KZ_ACDI] ? tu.v[KNOB_KZ_ACDI] : march_kz(n, ATX, ATY);
+        const int kz = march_kz(n, ATX, ATY, nbox);
         const dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, (n / kz) * nbox);
         if (c.mixed)
             hipLaunchKernelGGL(kg_acdi<float>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
@@ -2159,7 +2163,7 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
             hipLaunchKernelGGL(kg_acdi<double>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
     }
     {
-        const int kz = tu.v[KNOB_KZ_MOM] ? tu.v[KNOB_KZ_MOM] : march_kz(n, MTX, MTY);
+        const int kz = march_kz(n, MTX, MTY, nbox);
         if (c.mixed) {
             const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, (n / kz) * nbox);
             if (c.phiflag)
