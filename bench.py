@@ -141,6 +141,7 @@ def ensemble_leg(count=16, steps=(200, 800), inflight=2):
     t_one, out_one = serve.run_one_at_a_time(reqs)
     wall, out_ens, lat, res = serve.run_ensemble(reqs, follow_arrivals=True, inflight=inflight)
     same = all(np.array_equal(out_one[q], out_ens[q]) for q in out_one)
+    small = small_box_leg(steps=steps)
     return {"workload": f"{count} requests, N in {list(serve.SIZES)}, {steps[0]}-{steps[1]} steps, "
                         "half ACDI, mixed precision, Poisson arrivals 40/s",
             "cell_steps": cells, "one_at_a_time_s": round(t_one, 4), "ensemble_s": round(wall, 4),
@@ -149,7 +150,29 @@ def ensemble_leg(count=16, steps=(200, 800), inflight=2):
             "speedup": round(t_one / wall, 3), "step_latency_p50_ms": round(lat["p50_ms"], 4),
             "step_latency_p99_ms": round(lat["p99_ms"], 4), "steps_timed": lat["steps"],
             "same_alone_and_in_mix": bool(same), "status": sorted(set(r["status"] for r in res)),
-            "inflight": inflight}
+            "inflight": inflight, "small_boxes": small}
+
+
+def small_box_leg(count=32, steps=(200, 800)):
+    """A stream of small boxes only (N = 64, otherwise as the serving stream),
+    all submitted at once: one box at a time, the ensemble with each box on its
+    own stream (gang_size 1), and the ensemble stepping boxes of one class by
+    shared launches (gangs of 8)."""
+    import numpy as np
+    from mhit36_amd import serve
+    reqs = serve.c5_stream(count=count, steps=steps, sizes=(64,))
+    cells = serve.cell_steps(reqs)
+    serve.pretune(reqs)
+    t_one, out_one = serve.run_one_at_a_time(reqs)
+    out = {"workload": f"{count} requests, N = 64, {steps[0]}-{steps[1]} steps, half ACDI, mixed precision, "
+                       "submitted at once", "cell_steps": cells, "one_at_a_time_s": round(t_one, 4)}
+    for gang, key in ((1, "streams"), (8, "gangs")):
+        wall, o, lat, res = serve.run_ensemble(reqs, follow_arrivals=False, gang_size=gang)
+        out[key] = {"ensemble_s": round(wall, 4), "speedup": round(t_one / wall, 3),
+                    "throughput_gcell_steps_s": round(cells / wall / 1e9, 3),
+                    "step_latency_p50_ms": round(lat["p50_ms"], 4), "step_latency_p99_ms": round(lat["p99_ms"], 4),
+                    "same_alone_and_in_mix": bool(all(np.array_equal(out_one[q], o[q]) for q in out_one))}
+    return out
 
 
 def main():

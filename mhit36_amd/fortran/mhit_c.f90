@@ -73,6 +73,13 @@ module mhit_c
        integer(c_int), value :: nsteps
        type(mhit_diag), intent(out) :: d
      end function
+     ! Several contexts of one nx/precision/phiflag, one launch per pass.
+     integer(c_int) function mhit_step_batch(ctx, nbox, nsteps, d) bind(C, name="mhit_step_batch")
+       import :: c_int, c_ptr, mhit_diag
+       type(c_ptr), intent(in) :: ctx(*)
+       integer(c_int), value :: nbox, nsteps
+       type(mhit_diag), intent(out) :: d(*)
+     end function
      integer(c_int) function mhit_reset_time(ctx, tstart) bind(C, name="mhit_reset_time")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
