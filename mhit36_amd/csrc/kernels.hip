@@ -2007,7 +2007,7 @@ int knob_candidates(int knob, const Consts &c, const FftPlan &pl, int *out, int 
     const bool regs = plan_regs(pl);
     const int per_line = regs ? n / 8 : n / plan_rmin(pl);   // threads per line
     int k = 0;
-    for (int v = 1; v <= 256 && k < cap; v *= 2) {
+    for (int v = 1; v <= 1024 && k < cap; v *= 2) {
         bool ok;
         if (knob == KNOB_KZ_ACDI || knob == KNOB_KZ_MOM) {
             ok = v >= 2 && v <= n && n % v == 0;

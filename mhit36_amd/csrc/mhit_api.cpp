@@ -1067,8 +1067,8 @@ static int tune_impl(mhit_ctx *c, double *ms_h, double *ms_t)
     }
     for (int knob = 0; knob < KNOB_COUNT && rc == MHIT_OK; knob++) {
         if (knob == KNOB_KZ_ACDI && !c->kc.phiflag) continue;
-        int cand[8];
-        const int nc = knob_candidates(knob, c->kc, c->plan, cand, 8);
+        int cand[12];
+        const int nc = knob_candidates(knob, c->kc, c->plan, cand, 12);
         const int def = knob_default(knob, c->kc, c->plan);
         Tuning t = best;
         t.v[knob] = def;
@@ -1145,8 +1145,8 @@ extern "C" int mhit_set_tuning(mhit_ctx *c, const mhit_tuning *in)
     for (int q = 0; q < KNOB_COUNT; q++) {
         const int v = in->value[q];
         if (v == 0) continue;
-        int cand[8];
-        const int nc = knob_candidates(q, c->kc, c->plan, cand, 8);
+        int cand[12];
+        const int nc = knob_candidates(q, c->kc, c->plan, cand, 12);
         bool ok = false;
         for (int i = 0; i < nc; i++) ok = ok || cand[i] == v;
         if (!ok) return MHIT_ERR_ARG;
