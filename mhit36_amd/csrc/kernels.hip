@@ -1869,12 +1869,14 @@ void launch_debug_math(int op, size_t count, const double *x, double *y, const d
 static inline double F(const Consts &c) { return 8.0 * (double)c.n * c.n * c.n; }
 
 // Planes per block for the z-marching kernels: the largest of 32..2 dividing
-// n that still gives >= 2048 blocks (over `boxes` boxes launched together),
-// else the largest of 8..2 dividing n.
-static int march_kz(int n, int tx, int ty, int boxes = 1)
+// n that still gives >= `target` blocks (over `boxes` boxes launched
+// together), else the largest of 8..2 dividing n.  One box asks for 2048
+// blocks; a gang for 1024, since thin slabs cost halo planes (a block
+// recomputes kz + 4 psi planes for kz output planes).
+static int march_kz(int n, int tx, int ty, int boxes = 1, int target = 2048)
 {
     for (int kz = 32; kz > 1; kz >>= 1)
-        if (n % kz == 0 && boxes * (n / kz) * ((n + tx - 1) / tx) * ((n + ty - 1) / ty) >= 2048) return kz;
+        if (n % kz == 0 && boxes * (n / kz) * ((n + tx - 1) / tx) * ((n + ty - 1) / ty) >= target) return kz;
     for (int kz = 8; kz > 1; kz >>= 1)
         if (n % kz == 0) return kz;
     return 1;
@@ -2155,7 +2157,7 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
     // parallelism that thin slabs give one box); the FFT shapes are per line
     // or column and keep the box's tuned values.
     if (c.phiflag) {
-        const int kz = march_kz(n, ATX, ATY, nbox);
+        const int kz = march_kz(n, ATX, ATY, nbox, 1024);
         const dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, (n / kz) * nbox);
         if (c.mixed)
             hipLaunchKernelGGL(kg_acdi<float>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
@@ -2163,7 +2165,7 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
             hipLaunchKernelGGL(kg_acdi<double>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
     }
     {
-        const int kz = march_kz(n, MTX, MTY, nbox);
+        const int kz = march_kz(n, MTX, MTY, nbox, 1024);
         if (c.mixed) {
             const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, (n / kz) * nbox);
             if (c.phiflag)

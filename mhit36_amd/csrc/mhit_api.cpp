@@ -1594,26 +1594,28 @@ static int join_gang(mhit_ens *e, EnsReq *r)
     const Consts &k = r->ctx->kc;
     EnsGang *gg = nullptr;
     for (auto &g : e->gangs)
-        if ((int)g->members.size() < e->cfg.gang_size &&
+        if (g->ev[GANG_RING - 1] && (int)g->members.size() < e->cfg.gang_size &&
             (g->members.empty() || (g->nx == k.n && g->mixed == k.mixed && g->phiflag == k.phiflag))) {
             gg = g.get();
             if (!g->members.empty()) break;      // prefer a running gang of the class
         }
     if (!gg) {
-        auto g = std::make_unique<EnsGang>();
+        // Resources of a gang that fails half-way are released with the ensemble.
+        e->gangs.push_back(std::make_unique<EnsGang>());
+        EnsGang *g = e->gangs.back().get();
         const size_t bytes = sizeof(GangBox) * 2 * GANG_MAX * GANG_RING;
-        if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) return MHIT_ERR_HIP;
+        if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+            e->gangs.pop_back();
+            return MHIT_ERR_HIP;
+        }
         if (hipMalloc((void **)&g->dev, bytes) != hipSuccess ||
             hipHostMalloc((void **)&g->host, bytes, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
-            if (g->dev) (void)hipFree(g->dev);
-            (void)hipStreamDestroy(g->stream);
             return MHIT_ERR_OOM;
         }
         for (int q = 0; q < GANG_RING; q++)
             if (hipEventCreateWithFlags(&g->ev[q], hipEventDisableTiming) != hipSuccess) return MHIT_ERR_HIP;
-        gg = g.get();
-        e->gangs.push_back(std::move(g));
+        gg = g;
     }
     gg->nx = k.n;
     gg->mixed = k.mixed;
@@ -1801,7 +1803,13 @@ extern "C" int mhit_ens_progress(mhit_ens *e, int *n_active, int *n_queued)
             continue;
         }
         e->admitting--;
-        if (r->job_rc == MHIT_OK && r->ganged) r->job_rc = join_gang(e, r);
+        if (r->job_rc == MHIT_OK && r->ganged) {
+            r->job_rc = join_gang(e, r);
+            if (r->job_rc != MHIT_OK) {
+                mhit_destroy(r->ctx);
+                r->ctx = nullptr;
+            }
+        }
         if (r->job_rc == MHIT_OK) {
             r->admit_ms = ens_now(e);
             e->active.push_back(r);
@@ -1994,8 +2002,8 @@ extern "C" int mhit_ens_destroy(mhit_ens *e)
         for (hipEvent_t ev : g->ev)
             if (ev) (void)hipEventDestroy(ev);
         (void)hipStreamDestroy(g->stream);
-        (void)hipFree(g->dev);
-        (void)hipHostFree(g->host);
+        if (g->dev) (void)hipFree(g->dev);
+        if (g->host) (void)hipHostFree(g->host);
     }
     for (EnsSlot *sl : e->all_slots) (void)hipHostFree(sl);
     for (hipStream_t st : e->streams) (void)hipStreamDestroy(st);
