@@ -1400,7 +1400,9 @@ struct EnsGang {
     GangBox *host = nullptr;             // pinned staging, same layout
     hipEvent_t ev[GANG_RING] = {};
     bool ev_live[GANG_RING] = {};
+    hipGraphExec_t gx[GANG_RING][2] = {};   // AB2 step of the slot's members, per phi parity
     int slot = -1, parity = 0;
+    int steady = 0;                      // steps launched since the last upload
 };
 
 }  // namespace
@@ -1646,6 +1648,11 @@ static int gang_upload(EnsGang *g)
     }
     const int s = (g->slot + 1) % GANG_RING;
     if (g->ev_live[s]) HIPCHK(hipEventSynchronize(g->ev[s]));
+    for (hipGraphExec_t &x : g->gx[s])
+        if (x) {
+            (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
     GangBox *h = g->host + (size_t)s * 2 * GANG_MAX;
     for (size_t b = 0; b < g->members.size(); b++) {
         const EnsReq *r = g->members[b];
@@ -1664,7 +1671,39 @@ static int gang_upload(EnsGang *g)
                           hipMemcpyHostToDevice, g->stream));
     g->slot = s;
     g->parity = 0;
+    g->steady = 0;
     g->dirty = false;
+    return MHIT_OK;
+}
+
+// Launch one gang step.  Once a membership has run GANG_STEADY steps, its AB2
+// steps replay a graph captured per phi parity (Euler steps, which come only
+// right after a box joins, are launched directly).
+constexpr int GANG_STEADY = 4;
+static int gang_launch(mhit_ens *e, EnsGang *g, unsigned euler)
+{
+    const mhit_ctx *c0 = g->members[0]->ctx;
+    const GangBox *d = g->dev + (size_t)g->slot * 2 * GANG_MAX + (g->phiflag && g->parity ? GANG_MAX : 0);
+    const int m = (int)g->members.size();
+    if (euler || !e->cfg.use_graphs || g->steady < GANG_STEADY) {
+        launch_gang_step(d, m, euler, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
+        HIPCHK(hipGetLastError());
+        return MHIT_OK;
+    }
+    hipGraphExec_t &x = g->gx[g->slot][g->parity];
+    if (!x) {
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
+        launch_gang_step(d, m, 0u, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
+        HIPCHK(hipStreamEndCapture(g->stream, &gr));
+        hipError_t ei = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+            x = nullptr;
+            return MHIT_ERR_HIP;
+        }
+    }
+    HIPCHK(hipGraphLaunch(x, g->stream));
     return MHIT_OK;
 }
 
@@ -1689,10 +1728,9 @@ static int gang_progress(mhit_ens *e, EnsGang *g)
         unsigned euler = 0;
         for (size_t b = 0; b < g->members.size(); b++)
             if (g->members[b]->launched == 0) euler |= 1u << b;
-        const mhit_ctx *c0 = g->members[0]->ctx;
-        launch_gang_step(g->dev + (size_t)g->slot * 2 * GANG_MAX + (g->phiflag && g->parity ? GANG_MAX : 0),
-                         (int)g->members.size(), euler, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
-        HIPCHK(hipGetLastError());
+        int rc = gang_launch(e, g, euler);
+        if (rc) return rc;
+        g->steady++;
         g->parity ^= 1;
         for (EnsReq *r : g->members) r->launched++;
     }
@@ -1999,6 +2037,9 @@ extern "C" int mhit_ens_destroy(mhit_ens *e)
     for (auto &kv : e->all)
         if (kv.second->stream) e->streams.push_back(kv.second->stream);
     for (auto &g : e->gangs) {
+        for (auto &xs : g->gx)
+            for (hipGraphExec_t x : xs)
+                if (x) (void)hipGraphExecDestroy(x);
         for (hipEvent_t ev : g->ev)
             if (ev) (void)hipEventDestroy(ev);
         (void)hipStreamDestroy(g->stream);

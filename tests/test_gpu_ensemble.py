@@ -152,3 +152,24 @@ def test_gangs_take_and_release_boxes():
             for f, a in ref[q].items():
                 assert np.array_equal(ens.get_field(rid, f), a), (q, f)
         assert ens.latency()["steps"] == sum(steps) - len(steps)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_gang_graph_replays_match_alone(graphs):
+    """Long gang memberships: after a few steps a steady gang replays captured
+    graphs (one per phi parity); the membership changes as boxes finish (20,
+    30 and 40 steps), so graphs are captured again for the smaller gangs.
+    Every box still equals the same box alone."""
+    _, dx = ic.grid(64)
+    ps = [dict(nx=64, dt=(0.04 + 0.01 * q) * dx, mu=0.003 + 0.002 * q, phiflag=1, radius=0.6 + 0.1 * q,
+               sigma=0.05, gamma_factor=1.2, precision=1) for q in range(4)]
+    steps = [20, 30, 40, 40]
+    ref = [alone(p, n) for p, n in zip(ps, steps)]
+    with Ensemble(inflight=2, gang_size=4, use_graphs=graphs) as ens:
+        ids = [ens.submit(default_params(**p), nsteps=n) for p, n in zip(ps, steps)]
+        ens.wait_idle()
+        res = {r["id"]: r for r in ens.results()}
+        for q, rid in enumerate(ids):
+            assert res[rid]["status"] == 0 and res[rid]["steps_done"] == steps[q]
+            for f, a in ref[q].items():
+                assert np.array_equal(ens.get_field(rid, f), a), (q, f)
