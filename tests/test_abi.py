@@ -45,6 +45,20 @@ def test_struct_layouts():
     assert [f[0] for f in _lib.Params._fields_] == [f[0] for f in orc.OrcParams._fields_]
     assert ctypes.sizeof(_lib.Diag) == 4 * 4 + 6 * 8
     assert ctypes.sizeof(_lib.KStat) == 32 + 8 + 16
+    from mhit36_amd.ensemble import EnsConfig
+    # size_t budget, then max_active, inflight, use_graphs, autotune, gang_size, gang_max_nx
+    assert ctypes.sizeof(EnsConfig) == 8 + 6 * 4
+    assert [f[0] for f in EnsConfig._fields_][-2:] == ["gang_size", "gang_max_nx"]
+
+
+def test_step_batch_rejects_bad_arguments_before_any_device_call():
+    lib = _lib.load()
+    hs = (ctypes.c_void_p * 2)(None, None)
+    assert lib.mhit_step_batch(None, 1, 1, None) == _lib.MHIT_ERR_ARG
+    assert lib.mhit_step_batch(hs, 0, 1, None) == _lib.MHIT_ERR_ARG
+    assert lib.mhit_step_batch(hs, _lib.MHIT_BATCH_MAX + 1, 1, None) == _lib.MHIT_ERR_ARG
+    assert lib.mhit_step_batch(hs, 2, 1, None) == _lib.MHIT_ERR_ARG      # null contexts
+    assert lib.mhit_step_batch(hs, 1, -1, None) == _lib.MHIT_ERR_ARG
 
 
 def test_defaults_follow_input_inp():
