@@ -887,8 +887,8 @@ __device__ __forceinline__ void k_xfwd_body(uint3 blk, const T *__restrict__ us,
         int m = q == 0 ? 0 : n - q;
         double2 zq = lds[pl_ * lstride + q], zm = lds[pl_ * lstride + m];
         double a = zq.x, b = zq.y, cc = zm.x, d = zm.y;
-        spec[((size_t)k * n + j) * ld + q] = make_double2((a + cc) * 0.5, (b - d) * 0.5);
-        spec[((size_t)k * n + j + 1) * ld + q] = make_double2((b + d) * 0.5, (cc - a) * 0.5);
+        spec[spec_row(k, j, n, ld) + q] = make_double2((a + cc) * 0.5, (b - d) * 0.5);
+        spec[spec_row(k, j + 1, n, ld) + q] = make_double2((b + d) * 0.5, (cc - a) * 0.5);
     }
 }
 
@@ -914,7 +914,7 @@ __device__ __forceinline__ void k_ypass_body(uint3 blk, double2 *__restrict__ sp
     const int n = pl.n, nh = n / 2 + 1;
     const int q0 = blk.x * tkx, k = blk.y;
     const int nq = min(tkx, nh - q0);
-    double2 *base = spec + (size_t)k * n * ld + q0;
+    double2 *base = spec + spec_row(k, 0, n, ld) + q0;
     for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
         lds[r * tkx + ql] = ql < nq ? base[(size_t)r * ld + ql] : make_double2(0.0, 0.0);
@@ -950,7 +950,7 @@ __device__ __forceinline__ void k_zpoisson_body(uint3 blk, double2 *__restrict__
     const int n = pl.n, nh = n / 2 + 1;
     const int q0 = blk.x * tkx, j = blk.y;
     const int nq = min(tkx, nh - q0);
-    const size_t rs = (size_t)n * ld;
+    const size_t rs = spec_plane(n, ld);
     double2 *base = spec + (size_t)j * ld + q0;
     for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
@@ -1006,8 +1006,8 @@ __device__ __forceinline__ void k_xinv_body(uint3 blk, const double2 *__restrict
         int pl_ = e / n, q = e - pl_ * n;
         int pair = pair0 + pl_;
         int k = pair / half, j = 2 * (pair - k * half);
-        const double2 *s1 = spec + ((size_t)k * n + j) * ld;
-        const double2 *s2 = spec + ((size_t)k * n + j + 1) * ld;
+        const double2 *s1 = spec + spec_row(k, j, n, ld);
+        const double2 *s2 = spec + spec_row(k, j + 1, n, ld);
         double2 z;
         if (q == 0 || q == half) {
             z = make_double2(s1[q].x, s2[q].x);
@@ -1181,8 +1181,8 @@ __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
         int mq = q == 0 ? 0 : n - q;
         double2 zq = lds[rlds<true>(pq, q, lstride, 1)], zm = lds[rlds<true>(pq, mq, lstride, 1)];
         double a = zq.x, b = zq.y, cc = zm.x, d = zm.y;
-        nt_store2(&spec[((size_t)kk * n + jj) * ld + q], (a + cc) * 0.5, (b - d) * 0.5);
-        nt_store2(&spec[((size_t)kk * n + jj + 1) * ld + q], (b + d) * 0.5, (cc - a) * 0.5);
+        nt_store2(&spec[spec_row(kk, jj, n, ld) + q], (a + cc) * 0.5, (b - d) * 0.5);
+        nt_store2(&spec[spec_row(kk, jj + 1, n, ld) + q], (b + d) * 0.5, (cc - a) * 0.5);
     }
 }
 
@@ -1208,7 +1208,7 @@ __device__ __forceinline__ void k_ypass_r_body(uint3 blk, double2 *__restrict__ 
     const int q0 = blk.x * tkx, k = blk.y;
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const bool ok = q0 + line < nh;
-    double2 *base = spec + (size_t)k * n * ld + q0 + line;
+    double2 *base = spec + spec_row(k, 0, n, ld) + q0 + line;
     double2 v[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * ld] : make_double2(0.0, 0.0);
@@ -1242,7 +1242,7 @@ __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const int q = q0 + line;
     const bool ok = q < nh;
-    const size_t rs = (size_t)n * ld;
+    const size_t rs = spec_plane(n, ld);
     double2 *base = spec + (size_t)j * ld + q;
     double2 v[8];
 #pragma unroll
@@ -1289,8 +1289,8 @@ __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restri
     const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
     const int pair = blk.x * ppb + pl_;
     const int k = pair / half, j = 2 * (pair - k * half);
-    const double2 *s1 = spec + ((size_t)k * n + j) * ld;
-    const double2 *s2 = spec + ((size_t)k * n + j + 1) * ld;
+    const double2 *s1 = spec + spec_row(k, j, n, ld);
+    const double2 *s2 = spec + spec_row(k, j + 1, n, ld);
     double2 v[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) {

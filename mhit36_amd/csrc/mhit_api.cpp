@@ -416,7 +416,7 @@ static int create_impl(const mhit_params *p, mhit_ctx **out, Arena *arena, hipSt
         double **ph[] = {&f.phi, &f.phin};
         for (double **q : ph) TRY(dalloc(c, (void **)q, FT));
     }
-    TRY(dalloc(c, (void **)&f.spec, sizeof(double2) * (size_t)n * n * f.ld));
+    TRY(dalloc(c, (void **)&f.spec, sizeof(double2) * (size_t)n * spec_plane(n, f.ld)));
     TRY(dalloc(c, (void **)&f.lsum, sizeof(double) * 3 * (size_t)n * n));
     TRY(dalloc(c, (void **)&f.lmax, sizeof(double) * 3 * (size_t)n * n));
     TRY(dalloc(c, (void **)&f.psum, sizeof(double) * 3 * (size_t)n));
@@ -1110,7 +1110,8 @@ static int tune_impl(mhit_ctx *c, double *ms_h, double *ms_t)
     for (double *q : st) HIPCHK(hipMemsetAsync(q, 0, FT, c->stream));
     if (c->kc.phiflag) HIPCHK(hipMemsetAsync(c->f.phin, 0, FT, c->stream));
     HIPCHK(hipMemsetAsync(c->f.p, 0, n3 * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(c->f.spec, 0, sizeof(double2) * (size_t)c->kc.n * c->kc.n * c->f.ld, c->stream));
+    HIPCHK(hipMemsetAsync(c->f.spec, 0, sizeof(double2) * (size_t)c->kc.n * spec_plane(c->kc.n, c->f.ld),
+                         c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->tune = best;
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -1444,7 +1445,7 @@ static size_t ctx_bytes(const mhit_params *p)
     const size_t n = p->nx, n3 = n * n * n, F = n3 * 8;
     const size_t FT = n3 * (p->precision == MHIT_MIXED ? 4 : 8);
     const size_t ld = ((n / 2 + 1) + 7) / 8 * 8;
-    size_t b = 9 * FT + F + (p->phiflag ? 2 * FT : 0) + 16 * n * n * ld;
+    size_t b = 9 * FT + F + (p->phiflag ? 2 * FT : 0) + 16 * n * spec_plane((int)n, (int)ld);
     b += 8 * (6 * n * n + 6 * n + 4 * n) + 131072;
     return b;
 }
