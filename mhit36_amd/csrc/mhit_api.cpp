@@ -1684,14 +1684,15 @@ constexpr int GANG_STEADY = 4;
 static int gang_launch(mhit_ens *e, EnsGang *g, unsigned euler)
 {
     const mhit_ctx *c0 = g->members[0]->ctx;
-    const GangBox *d = g->dev + (size_t)g->slot * 2 * GANG_MAX + (g->phiflag && g->parity ? GANG_MAX : 0);
+    const int par = g->phiflag ? g->parity : 0;     // without phi both sets are the same
+    const GangBox *d = g->dev + (size_t)g->slot * 2 * GANG_MAX + (par ? GANG_MAX : 0);
     const int m = (int)g->members.size();
     if (euler || !e->cfg.use_graphs || g->steady < GANG_STEADY) {
         launch_gang_step(d, m, euler, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
         HIPCHK(hipGetLastError());
         return MHIT_OK;
     }
-    hipGraphExec_t &x = g->gx[g->slot][g->parity];
+    hipGraphExec_t &x = g->gx[g->slot][par];
     if (!x) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
