@@ -23,6 +23,9 @@
  *   dissipation ........... post/compute_diss/main.f90:61-119 + module.f90:5 (REAL(4) pi)
  *                                                            pinned: val/ series, t=0 rows
  *   orc_log / orc_tanh .... own implementations (see below) pinned: libm within 1 ulp (log), 3 ulp (tanh)
+ *   orc_drop_count ........ post/DropCountSize/get_interface.f90:11-47, flood_fill.f90
+ *                                                            pinned: constructed contacts/wrap/threshold,
+ *                                                            scipy.ndimage.label on a periodic tiling
  */
 #include "mhit_oracle.h"
 
