@@ -1,6 +1,8 @@
 ! main.f90 -- MHIT36 driver over libmhit (the shape of multi/main.f90).
 !
-!   mhit36 [nx]      (nx defaults to 128, the reference's module.f90:2)
+!   mhit36 [nx [phiflag [precision]]]
+!     nx defaults to 128 (the reference's module.f90:2), phiflag to 1,
+!     precision to 0 (FP64; 1 = mixed: FP32 state and stencils, FP64 Poisson).
 !
 ! Reads ./input.inp (readinput.f90), initialises the box (Taylor-Green /
 ! spherical drop, ./input/*.dat when inflow/inphi = 1, or ./output/*_tstart.dat
@@ -18,7 +20,7 @@ program mhit36
   integer(c_int) :: rc, nx, t
   character(len=64) :: arg
   character(len=256) :: fname
-  integer :: phiflag
+  integer :: phiflag, precision
 
   nx = 128
   if (command_argument_count() >= 1) then
@@ -31,8 +33,15 @@ program mhit36
      read(arg, *) phiflag
   end if
 
+  precision = 0
+  if (command_argument_count() >= 3) then
+     call get_command_argument(3, arg)
+     read(arg, *) precision
+  end if
+
   call mhit_check(mhit_read_input(cstr('input.inp'), nx, p), 'readinput')
   p%phiflag = phiflag
+  p%precision = precision
   call mhit_check(mhit_derive(p, cs), 'derive')
   write(*,*) "----------------------------------------------"
   write(*,*) "MHIT36 timestep on MI355X (libmhit)"
@@ -51,6 +60,7 @@ program mhit36
   write(*,*) "Eps     ", cs%eps
   write(*,*) "Lx      ", cs%lx
   write(*,*) "dx", cs%dx
+  write(*,*) "Precision (0 FP64, 1 mixed)", p%precision
 
   call mhit_check(mhit_create(p, ctx), 'create')
 

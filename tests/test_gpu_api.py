@@ -137,6 +137,28 @@ def test_fortran_driver_end_to_end(tmp_path):
         assert nd == orc.drop_count(phi)[0] == 1
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "mhit36_amd", "fortran", "mhit36")),
+                    reason="Fortran driver not built (amdflang missing)")
+def test_fortran_driver_mixed_precision(tmp_path):
+    """`mhit36 16 1 1` runs the mixed-precision step: its dumps equal the
+    fields of the same box stepped through the Python binding."""
+    n = 16
+    lines = open(os.path.join(ROOT, "tests", "golden", "input.inp")).read().splitlines()
+    lines[2] = "4           ! Final time step"
+    lines[3] = "4           ! Saving frequency"
+    (tmp_path / "input.inp").write_text("\n".join(lines) + "\n")
+    (tmp_path / "output").mkdir()
+    exe = os.path.join(ROOT, "mhit36_amd", "fortran", "mhit36")
+    r = subprocess.run([exe, str(n), "1", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with Solver(nx=n, phiflag=1, precision=1) as s:
+        s.init_fields()
+        s.step(4)
+        for f, name in ((1, "u"), (2, "v"), (3, "w"), (5, "phi")):
+            raw = np.fromfile(tmp_path / "output" / f"{name}_00000004.dat", dtype="<f8").reshape((n, n, n), order="F")
+            assert np.array_equal(raw, s.get_field(f)), name
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_checkpoint_resume_is_bitwise(tmp_path, precision):
     """mhit_save_state / mhit_load_state: 4 steps, checkpoint, a new context
