@@ -1,0 +1,42 @@
+"""The reference's own end-to-end validation, run on the device: Taylor-Green
+vortex at Re=1600 (nu=6.25e-4, dt=2e-4, Euler first step then AB2), the
+dissipation of post/compute_diss (REAL(4) pi, module.f90:5) every 1000 steps,
+against the series the reference ships in val/ (val/comparison_diss.m:5,10;
+the rows are the committed fixture tests/golden/val_tgv_dissipation.json).
+
+The device follows cudec_N_ab to ~1e-12 relative over the whole run (50 000
+steps to t = 10 at 128^3) while the Euler-labelled series is 1e-5..5e-3 away,
+so the test pins the scheme as well as the arithmetic.  The residual is the
+order of the dissipation sum (device tree against compute_diss's loops) and
+of the FFT's rounding, not a drift: it does not grow with t."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mhit36_amd import Solver
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "val_tgv_dissipation.json")))["series"]
+PI4 = float(np.float32(3.14159265358979))     # post/compute_diss/module.f90:5
+NU = 0.000625                                 # post/compute_diss/main.f90:11
+
+
+@pytest.mark.parametrize("n,samples", [(128, 50), (256, 20)])
+def test_tgv_re1600_follows_the_reference_series(n, samples):
+    dxi = 1.0 / ((2 * PI4) / (n - 1))          # compute_diss/main.f90:8-10
+    with Solver(nx=n, mu=NU, dt=2e-4, phiflag=0, inflow=0) as s:
+        s.init_fields()
+        for q in range(samples + 1):
+            if q:
+                d = s.step(1000)
+                assert d.steps_done == 1000 and d.status == 0
+            eps = s.diagnostics(nu=NU, dxi=dxi)["diss"]
+            t, ref = GOLD[f"cudec_{n}_ab"][q]
+            assert abs(t - 0.2 * q) < 1e-9
+            assert abs(eps - ref) <= 1e-11 * ref, (n, t, eps, ref)
+            if q >= 5:
+                _, other = GOLD[f"cudec_{n}_eulero"][q]
+                assert abs(eps - other) > 1e-6 * other, (n, t)
