@@ -58,19 +58,28 @@ __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n
 // FP64: the project's own table-driven log/tanh (bit-identical to the
 // oracle; `tab` is the block's LDS copy of the table) and IEEE division.
 // FP32 (the mixed-precision pass, compared with FP64 by tolerance): the
-// hardware log2/exp2 approximations and a reciprocal-based quotient (a few
-// ulp of FP32, far below the FP32 storage error of the state).
+// hardware log2/exp2 approximations (v_log_f32, v_exp_f32) and quotients as
+// a product with the hardware reciprocal (v_rcp_f32): a few ulp of FP32, far
+// below the FP32 storage error of the state, in 1-2 instructions where the
+// correctly rounded forms take 8-10.  (HIP's __logf and __fdividef are the
+// correctly rounded forms unless fast math is on.)  The arguments are
+// positive normal numbers or moderate in size: log's is (phi + 1e-16) /
+// (1 - phi + 1e-16) for phi clamped to [0, 1], the divisors 1 + e and
+// |grad psi| + 1e-16.
 __device__ __forceinline__ double tlog(double x, const double *tab) { return dlog(x, tab); }
-__device__ __forceinline__ float tlog(float x, const double *) { return __logf(x); }
+__device__ __forceinline__ float tlog(float x, const double *)
+{
+    return __builtin_amdgcn_logf(x) * 0.6931471805599453f;       // log2(x) ln 2
+}
 __device__ __forceinline__ double ttanh(double x, const double *tab) { return dtanh(x, tab); }
 __device__ __forceinline__ float ttanh(float x, const double *)
 {
-    const float e = __expf(-2.0f * __builtin_fabsf(x));          // tanh|x| = (1 - e)/(1 + e)
-    const float t = __fdividef(1.0f - e, 1.0f + e);
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * __builtin_fabsf(x));   // exp(-2|x|)
+    const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);                       // tanh|x|
     return x < 0.0f ? -t : t;
 }
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float tdiv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 template <typename T>
 __device__ __forceinline__ void k_acdi_body(uint3 blk,
