@@ -58,8 +58,8 @@ __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n
 // FP64: the project's own table-driven log/tanh (bit-identical to the
 // oracle; `tab` is the block's LDS copy of the table) and IEEE division.
 // FP32 (the mixed-precision pass, compared with FP64 by tolerance): the
-// hardware log2/exp2 approximations (v_log_f32, v_exp_f32) and quotients as
-// a product with the hardware reciprocal (v_rcp_f32): a few ulp of FP32, far
+// hardware log2/exp2/sqrt approximations (v_log_f32, v_exp_f32, v_sqrt_f32)
+// and quotients as a product with the hardware reciprocal (v_rcp_f32): a few ulp of FP32, far
 // below the FP32 storage error of the state, in 1-2 instructions where the
 // correctly rounded forms take 8-10.  (HIP's __logf and __fdividef are the
 // correctly rounded forms unless fast math is on.)  The arguments are
@@ -80,6 +80,8 @@ __device__ __forceinline__ float ttanh(float x, const double *)
 }
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename T>
 __device__ __forceinline__ void k_acdi_body(uint3 blk,
@@ -153,7 +155,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 T gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
                 T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
                 T gz = psiL[psp][pc] - psiL[psm][pc];
-                T normod = tdiv((T)1.0, (T)(sqrt((gx * gx + gy * gy) + gz * gz) + tiny));
+                T normod = tdiv((T)1.0, tsqrt((gx * gx + gy * gy) + gz * gz) + tiny);
                 T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi, tabL);
                 T a = (T)0.25 * ((T)1.0 - th * th);
                 sxyL[sxy][0][e] = a * (gx * normod);
