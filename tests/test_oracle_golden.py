@@ -59,3 +59,12 @@ def test_val_t02_128_ab_1000_steps():
     assert abs(eps - ref_ab) / ref_ab < 1e-12, (eps, ref_ab)
     _, ref_eu = GOLD["series"]["cudec_128_eulero"][1]
     assert abs(eps - ref_eu) / ref_eu > 1e-6
+
+
+def test_val_fixture_holds_every_row():
+    """The fixture carries each val/ series whole: one sample per 1000 steps
+    (t = 0.2 q), 51 rows to t = 10, positive dissipation."""
+    for name, rows in GOLD["series"].items():
+        assert len(rows) == 51, name
+        for q, (t, eps) in enumerate(rows):
+            assert abs(t - 0.2 * q) < 1e-9 and eps > 0, (name, q)
