@@ -150,7 +150,9 @@ int mhit_step(mhit_ctx *ctx, int nsteps, mhit_diag *diag);
 #define MHIT_BATCH_MAX 16
 int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_diag *diag);
 
-/* Set the step counter and make the next step an Euler step (restart). */
+/* Restart (main.f90:318-323): set the step counter, make the next step an
+ * Euler step and set gumax to 1.0, the value behind the lagged gamma of a
+ * run's first step (main.f90:321, 362).  Clears a CFL/NaN halt. */
 int mhit_reset_time(mhit_ctx *ctx, int tstart);
 
 /* writefield (readwrite.f90:1-98): raw FP64, Fortran order, no header, to
@@ -168,7 +170,8 @@ int mhit_diagnostics(mhit_ctx *ctx, double dxi, double nu, mhit_stats *out);
 int mhit_kernel_timing(mhit_ctx *ctx, int enable);
 int mhit_kernel_stats(mhit_ctx *ctx, mhit_kstat *out, int max_entries, int *n_entries);
 
-/* Device memory held by the context, bytes. */
+/* Device memory held by the context, bytes (with the drop-count workspace
+ * once mhit_drop_count has made it). */
 int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
 
 /* Checkpoint and resume of the exact device state (no counterpart in the
@@ -176,8 +179,10 @@ int mhit_device_bytes(mhit_ctx *ctx, size_t *bytes);
  * Euler step, main.f90:243-245, 323-326): u, v, w as stored with their pending
  * means, rhs_o, phi, p, the scalars (umax behind the lagged gamma, halt
  * state), the time index and the AB2 state.  A context loaded from a state
- * file continues bit for bit as if it had not stopped.  load needs a context
- * of the same nx, precision and phiflag (MHIT_ERR_ARG otherwise). */
+ * file continues bit for bit as if it had not stopped.  load needs a file
+ * written by this ABI version for a context of the same nx, precision and
+ * phiflag (MHIT_ERR_ARG otherwise) and of exactly the state's length
+ * (MHIT_ERR_IO otherwise). */
 int mhit_save_state(mhit_ctx *ctx, const char *path);
 int mhit_load_state(mhit_ctx *ctx, const char *path);
 
@@ -212,8 +217,10 @@ int mhit_set_tuning(mhit_ctx *ctx, const mhit_tuning *in);
  * calculate_deq.f90): cells with phi >= 0.5, 26-connected with periodic wrap;
  * drops in raster order of their first cell (x fastest, then y, then z);
  * volume[q] in cells, deq[q] = dx (6 volume/pi)^(1/3).  *n_drops is the total
- * count; the first max_drops entries are filled.  Needs phiflag = 1 and
- * 4 N^3 bytes of device memory for the labels (allocated per call). */
+ * count; the first max_drops entries are filled.  Needs phiflag = 1.  The
+ * context's first call allocates a labelling workspace (4 N^3 bytes of labels
+ * plus 12 bytes per possible drop, N^3/8 + 1 of them, at most 2^24) that the
+ * context keeps until mhit_destroy; later calls allocate nothing. */
 int mhit_drop_count(mhit_ctx *ctx, int max_drops, int *n_drops, double *volume, double *deq);
 
 /* Diagnostics (no counterpart in the reference): evaluate the device's own
@@ -256,7 +263,9 @@ typedef struct mhit_ens_result {
 int mhit_ens_create(const mhit_ens_config *cfg, mhit_ens **ens);
 int mhit_ens_destroy(mhit_ens *ens);
 /* Queue a box: parameters, optional initial u, v, w (and phi) interior fields
- * (NULL: the built-in IC of mhit_init_fields), and the number of steps. */
+ * (NULL: the built-in IC of mhit_init_fields), and the number of steps.
+ * u, v and w come all three or not at all; p->inflow = 1 needs them and
+ * p->phiflag with p->inphi = 1 needs phi (MHIT_ERR_ARG otherwise). */
 int mhit_ens_submit(mhit_ens *ens, const mhit_params *p, const double *u, const double *v, const double *w,
                     const double *phi, int nsteps, long long *id);
 /* One non-blocking scheduling pass: admit, retire finished steps, queue new ones. */

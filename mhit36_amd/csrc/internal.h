@@ -137,11 +137,32 @@ void launch_debug_math(int op, size_t count, const double *x, double *y, const d
 // Drop counting: label the phi >= 0.5 cells (26-connected, periodic) by the
 // smallest linear index of their set; list the roots; count cells per root
 // (roots sorted ascending).
+// The workspace: labels (n^3), the sorted root list and its volumes (cap
+// each), per-chunk counts and offsets, the root count.
+struct CclWork {
+    unsigned *lab, *roots, *counts, *offs, *nroots;
+    unsigned long long *vol;
+    unsigned cap;
+};
+// Most drops a box of n3 cells can hold: components need an empty cell between
+// them in every direction, so at most one cell in eight starts one.
+constexpr size_t ccl_cap(size_t n3)
+{
+    return n3 / 8 + 1 < ((size_t)1 << 24) ? n3 / 8 + 1 : ((size_t)1 << 24);
+}
+// The root list is built over CCL_CHUNKS contiguous chunks of cells.
+constexpr int CCL_CHUNKS = 4096;
+constexpr size_t ccl_up(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t ccl_workspace_bytes(int n)
+{
+    return ccl_up((size_t)n * n * n * 4) + ccl_up(ccl_cap((size_t)n * n * n) * 4) +
+           ccl_up(ccl_cap((size_t)n * n * n) * 8) + 2 * ccl_up(CCL_CHUNKS * 4) + 256;
+}
+CclWork ccl_carve(void *base, int n);
 void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream_t s);
-void launch_ccl_roots(const Consts &c, const unsigned *lab, unsigned *roots, unsigned *nroots, unsigned cap,
-                      hipStream_t s);
-void launch_ccl_volume(const Consts &c, const unsigned *lab, const unsigned *roots, int nr, unsigned long long *vol,
-                       hipStream_t s);
+// Label, list the roots in raster order, count each drop's cells.  Afterwards
+// *w.nroots holds the drop count; w.vol[0..count) the volumes when count <= cap.
+void launch_ccl_count(const Fields &f, const Consts &c, const CclWork &w, hipStream_t s);
 // The log/expm1 table of devmath.h, MATH_TAB doubles (host).
 void build_math_table(double *out);
 void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s);

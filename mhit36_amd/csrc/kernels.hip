@@ -1795,26 +1795,104 @@ __global__ void __launch_bounds__(BLK) k_ccl_flatten(unsigned *__restrict__ lab,
         if (lab[x] != CCL_NONE) lab[x] = ccl_find(lab, (unsigned)x);
 }
 
-__global__ void __launch_bounds__(BLK) k_ccl_roots(const unsigned *__restrict__ lab, size_t n3,
-                                                   unsigned *__restrict__ roots, unsigned *__restrict__ nroots,
-                                                   unsigned cap)
+// Roots in raster order, without a sort: the cells are cut into CCL_CHUNKS
+// contiguous chunks; pass 1 counts the roots (cells labelled by their own
+// index) of each chunk, pass 2 turns the counts into chunk offsets and the
+// total, pass 3 writes each chunk's roots at its offset in cell order (one
+// block per chunk, a 256-cell tile at a time, ranks from wavefront ballots).
+static_assert(CCL_CHUNKS % BLK == 0, "the scan gives each thread CCL_CHUNKS / BLK chunks");
+
+__device__ __forceinline__ size_t ccl_chunk_begin(size_t n3, int b)
 {
-    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK)
-        if (lab[x] == (unsigned)x) {
-            const unsigned q = atomicAdd(nroots, 1u);
-            if (q < cap) roots[q] = (unsigned)x;
-        }
+    return (size_t)(((unsigned long long)n3 * (unsigned)b) / CCL_CHUNKS);
 }
 
-// Cells per drop: position of the cell's label among the sorted roots.
+__global__ void __launch_bounds__(BLK) k_ccl_count(const unsigned *__restrict__ lab, size_t n3,
+                                                   unsigned *__restrict__ counts)
+{
+    __shared__ unsigned part[BLK / 64];
+    const size_t lo = ccl_chunk_begin(n3, blockIdx.x), hi = ccl_chunk_begin(n3, blockIdx.x + 1);
+    unsigned c = 0;
+    for (size_t x = lo + threadIdx.x; x < hi; x += BLK) c += lab[x] == (unsigned)x;
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int q = 0; q < BLK / 64; q++) t += part[q];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// One block: exclusive scan of the CCL_CHUNKS counts, total into *nroots.
+__global__ void __launch_bounds__(BLK) k_ccl_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ offs,
+                                                  unsigned *__restrict__ nroots)
+{
+    constexpr int PER = CCL_CHUNKS / BLK;
+    __shared__ unsigned tsum[BLK];
+    unsigned loc[PER], run = 0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        loc[q] = run;
+        run += counts[threadIdx.x * PER + q];
+    }
+    tsum[threadIdx.x] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int t = 0; t < BLK; t++) {
+            const unsigned v = tsum[t];
+            tsum[t] = acc;
+            acc += v;
+        }
+        *nroots = acc;
+    }
+    __syncthreads();
+    const unsigned base = tsum[threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < PER; q++) offs[threadIdx.x * PER + q] = base + loc[q];
+}
+
+__global__ void __launch_bounds__(BLK) k_ccl_place(const unsigned *__restrict__ lab, size_t n3,
+                                                   const unsigned *__restrict__ offs, unsigned *__restrict__ roots,
+                                                   unsigned cap)
+{
+    __shared__ unsigned wcnt[BLK / 64];
+    const size_t lo = ccl_chunk_begin(n3, blockIdx.x), hi = ccl_chunk_begin(n3, blockIdx.x + 1);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned pos = offs[blockIdx.x];
+    for (size_t t0 = lo; t0 < hi; t0 += BLK) {
+        const size_t x = t0 + threadIdx.x;
+        const bool r = x < hi && lab[x] == (unsigned)x;
+        const unsigned long long m = __ballot(r);
+        const unsigned below = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[wv] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned before = 0, total = 0;
+        for (int q = 0; q < BLK / 64; q++) {
+            before += q < wv ? wcnt[q] : 0u;
+            total += wcnt[q];
+        }
+        const unsigned at = pos + before + below;
+        if (r && at < cap) roots[at] = (unsigned)x;
+        pos += total;
+        __syncthreads();
+    }
+}
+
+// Cells per drop: position of the cell's label among the sorted roots.  Runs
+// only when every root fit the list (*nroots <= cap).
 __global__ void __launch_bounds__(BLK) k_ccl_volume(const unsigned *__restrict__ lab, size_t n3,
-                                                    const unsigned *__restrict__ roots, int nr,
+                                                    const unsigned *__restrict__ roots,
+                                                    const unsigned *__restrict__ nroots, unsigned cap,
                                                     unsigned long long *__restrict__ vol)
 {
+    const unsigned nr = *nroots;
+    if (nr == 0 || nr > cap) return;
     for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < n3; x += (size_t)gridDim.x * BLK) {
         const unsigned l = lab[x];
         if (l == CCL_NONE) continue;
-        int lo = 0, hi = nr - 1;
+        int lo = 0, hi = (int)nr - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (roots[mid] < l) lo = mid + 1;
@@ -1841,7 +1919,7 @@ void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *ou
 }
 
 // Grid of a grid-stride pass over n3 cells: enough blocks to fill the chip.
-static unsigned ccl_grid(size_t n3)
+unsigned ccl_grid(size_t n3)
 {
     const size_t b = nblocks(n3, BLK);
     return (unsigned)(b < 65536 ? b : 65536);
@@ -1857,18 +1935,42 @@ void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream
     hipLaunchKernelGGL(k_ccl_flatten, dim3(g), dim3(BLK), 0, s, lab, n3);
 }
 
-void launch_ccl_roots(const Consts &c, const unsigned *lab, unsigned *roots, unsigned *nroots, unsigned cap,
-                      hipStream_t s)
+__global__ void __launch_bounds__(BLK) k_fill_u32(unsigned *__restrict__ p, size_t count, unsigned v)
 {
-    const size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_ccl_roots, dim3(ccl_grid(n3)), dim3(BLK), 0, s, lab, n3, roots, nroots, cap);
+    for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < count; x += (size_t)gridDim.x * BLK) p[x] = v;
 }
 
-void launch_ccl_volume(const Consts &c, const unsigned *lab, const unsigned *roots, int nr, unsigned long long *vol,
-                       hipStream_t s)
+CclWork ccl_carve(void *base, int n)
+{
+    const size_t n3 = (size_t)n * n * n;
+    const size_t cap = ccl_cap(n3);
+    auto up = ccl_up;
+    char *p = (char *)base;
+    CclWork w;
+    w.lab = (unsigned *)p;                   p += up(n3 * 4);
+    w.roots = (unsigned *)p;                 p += up(cap * 4);
+    w.vol = (unsigned long long *)p;         p += up(cap * 8);
+    w.counts = (unsigned *)p;                p += up(CCL_CHUNKS * 4);
+    w.offs = (unsigned *)p;                  p += up(CCL_CHUNKS * 4);
+    w.nroots = (unsigned *)p;
+    w.cap = (unsigned)cap;
+    return w;
+}
+
+void launch_ccl_count(const Fields &f, const Consts &c, const CclWork &w, hipStream_t s)
 {
     const size_t n3 = (size_t)c.n * c.n * c.n;
-    hipLaunchKernelGGL(k_ccl_volume, dim3(ccl_grid(n3)), dim3(BLK), 0, s, lab, n3, roots, nr, vol);
+    const unsigned g = ccl_grid(n3);
+    launch_ccl_label(f, c, w.lab, s);
+    hipLaunchKernelGGL(k_ccl_count, dim3(CCL_CHUNKS), dim3(BLK), 0, s, w.lab, n3, w.counts);
+    hipLaunchKernelGGL(k_ccl_scan, dim3(1), dim3(BLK), 0, s, w.counts, w.offs, w.nroots);
+    hipLaunchKernelGGL(k_ccl_place, dim3(CCL_CHUNKS), dim3(BLK), 0, s, w.lab, n3, w.offs, w.roots, w.cap);
+    // volumes are counted by atomics: clear them first (a kernel, not a memset,
+    // so the sequence may sit in a captured graph)
+    const size_t nv = w.cap * 2;             // the u64 counters as u32 words
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(nv, BLK) < 4096 ? nblocks(nv, BLK) : 4096), dim3(BLK), 0, s,
+                       (unsigned *)w.vol, nv, 0u);
+    hipLaunchKernelGGL(k_ccl_volume, dim3(g), dim3(BLK), 0, s, w.lab, n3, w.roots, w.nroots, w.cap, w.vol);
 }
 
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s)

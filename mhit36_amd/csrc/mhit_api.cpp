@@ -110,6 +110,8 @@ struct mhit_ctx {
     std::vector<PendingTiming> pending;
     std::map<std::string, KStat> stats;
     GangBox *gang = nullptr;     // device descriptors of mhit_step_batch (2 x GANG_MAX)
+    void *ccl_ws = nullptr;      // drop-count workspace, made by the first mhit_drop_count
+    size_t ccl_bytes = 0;
 };
 
 #define HIPCHK(expr)                                                     \
@@ -293,6 +295,9 @@ static void free_all(mhit_ctx *c)
         else (void)hipFree(p);
     }
     c->allocs.clear();
+    if (c->ccl_ws) (void)hipFree(c->ccl_ws);
+    c->ccl_ws = nullptr;
+    c->ccl_bytes = 0;
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -565,11 +570,28 @@ extern "C" int mhit_init_fields(mhit_ctx *c)
     return MHIT_OK;
 }
 
+// A restart (main.f90:318-323): the step counter, an Euler next step, and
+// gumax = 1.0, the value the reference starts every run with (it sets the
+// lagged gamma of the first step, main.f90:321, 362).  The pending means of
+// the stored u, v, w stay, and so does the step count of the device
+// scalars.  A halt (CFL or NaN stop) is cleared: the restarted run steps
+// from the fields as they are, as the reference's restart would.
 extern "C" int mhit_reset_time(mhit_ctx *c, int tstart)
 {
     if (!c) return MHIT_ERR_ARG;
+    Scalars s;
+    int rc = read_scalars(c, &s);
+    if (rc) return rc;
+    s.gumax = 1.0;
+    s.cou = 0.0;
+    s.halted = 0;
+    s.status = 0;
+    HIPCHK(hipMemcpyAsync(c->f.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->t = tstart;
     c->nsteps_since_init = 0;
+    c->halted = 0;
+    c->halt_status = 0;
     return MHIT_OK;
 }
 
@@ -920,13 +942,24 @@ extern "C" int mhit_load_state(mhit_ctx *c, const char *path)
         fclose(fp);
         return MHIT_ERR_IO;
     }
+    if (h.abi != MHIT_ABI_VERSION) {
+        fclose(fp);
+        return MHIT_ERR_ARG;                                  // written with another header/scalars layout
+    }
     if (h.nx != c->kc.n || h.phiflag != c->kc.phiflag || h.precision != (c->kc.mixed ? MHIT_MIXED : MHIT_FP64)) {
         fclose(fp);
         return MHIT_ERR_ARG;                                  // a state of another kind of box
     }
+    const auto bufs = state_buffers(c);
+    size_t want = sizeof(h);
+    for (auto &b : bufs) want += b.second;
+    if (fseek(fp, 0, SEEK_END) != 0 || ftell(fp) != (long)want || fseek(fp, (long)sizeof(h), SEEK_SET) != 0) {
+        fclose(fp);
+        return MHIT_ERR_IO;                                   // truncated, or longer than this box's state
+    }
     std::vector<char> buf;
     int rc = MHIT_OK;
-    for (auto &b : state_buffers(c)) {
+    for (auto &b : bufs) {
         buf.resize(b.second);
         if (fread(buf.data(), 1, b.second, fp) != b.second) { rc = MHIT_ERR_IO; break; }
         if (hipMemcpyAsync(b.first, buf.data(), b.second, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -1183,67 +1216,63 @@ extern "C" int mhit_diagnostics(mhit_ctx *c, double dxi, double nu, mhit_stats *
 }
 
 // post/DropCountSize (get_interface.f90:11-47, flood_fill.f90:1-236,
-// calculate_deq.f90:1-40) on the context's phi.
-extern "C" int mhit_drop_count(mhit_ctx *c, int max_drops, int *n_drops, double *volume, double *deq)
+// calculate_deq.f90:1-40) on the context's phi, in the workspace `w`: count,
+// then the volumes of the first min(count, max_drops) drops in raster order
+// of their first cell.  The volumes come back through `volume` itself (a u64
+// count per entry, converted in place), so the host side allocates nothing.
+static int drop_count_in(mhit_ctx *c, const CclWork &w, int max_drops, int *n_drops, double *volume, double *deq)
+{
+    hipStream_t st = c->stream;
+    launch_ccl_count(c->f, c->kc, w, st);
+    HIPCHK(hipGetLastError());
+    unsigned nr = 0;
+    HIPCHK(hipMemcpyAsync(&nr, w.nroots, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (nr > w.cap) return MHIT_ERR_UNSUPPORTED;                       // more drops than the root list holds
+    const int k = (int)std::min<unsigned>(nr, (unsigned)max_drops);
+    static_assert(sizeof(unsigned long long) == sizeof(double), "volumes staged in the output array");
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(volume, w.vol, (size_t)k * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    // deq = dx (6/pi vol)^(1/3), pi = 4 atan(1), dx = lx/(nx-1) (calculate_deq.f90:31; module.f90:3-5)
+    const double n6opi = 6.0 / (4.0 * atan(1.0));
+    for (int q = 0; q < k; q++) {
+        unsigned long long cnt;
+        memcpy(&cnt, &volume[q], sizeof(cnt));
+        volume[q] = (double)cnt;
+        deq[q] = c->cs.dx * pow(n6opi * volume[q], 1.0 / 3.0);
+    }
+    *n_drops = (int)nr;
+    return MHIT_OK;
+}
+
+static int drop_count_args(mhit_ctx *c, int max_drops, int *n_drops, double *volume, double *deq)
 {
     if (!c || !n_drops || max_drops < 0 || (max_drops > 0 && (!volume || !deq))) return MHIT_ERR_ARG;
     if (!c->kc.phiflag) return MHIT_ERR_ARG;
-    const int n = c->kc.n;
-    const size_t n3 = (size_t)n * n * n;
-    if (n3 >= (size_t)0xffffffffu) return MHIT_ERR_BAD_N;          // 32-bit labels
-    const unsigned cap = (unsigned)std::min<size_t>(n3 / 8 + 1, (size_t)1 << 24);
-    unsigned *lab = nullptr, *roots = nullptr, *nroots = nullptr;
-    unsigned long long *vol = nullptr;
-    int rc = MHIT_OK;
-    if (hipMalloc((void **)&lab, n3 * sizeof(unsigned)) != hipSuccess ||
-        hipMalloc((void **)&roots, (size_t)cap * sizeof(unsigned)) != hipSuccess ||
-        hipMalloc((void **)&nroots, sizeof(unsigned)) != hipSuccess)
-        rc = MHIT_ERR_OOM;
-    hipStream_t st = c->stream;
-    unsigned nr = 0;
-    std::vector<unsigned> hroots;
-    if (rc == MHIT_OK) {
-        launch_ccl_label(c->f, c->kc, lab, st);
-        if (hipMemsetAsync(nroots, 0, sizeof(unsigned), st) != hipSuccess) rc = MHIT_ERR_HIP;
-        launch_ccl_roots(c->kc, lab, roots, nroots, cap, st);
-        if (rc == MHIT_OK && (hipGetLastError() != hipSuccess ||
-                              hipMemcpyAsync(&nr, nroots, sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
-                              hipStreamSynchronize(st) != hipSuccess))
-            rc = MHIT_ERR_HIP;
-    }
-    if (rc == MHIT_OK && nr > cap) rc = MHIT_ERR_UNSUPPORTED;           // more drops than the root list holds
-    if (rc == MHIT_OK && nr > 0) {
-        hroots.resize(nr);
-        if (hipMemcpyAsync(hroots.data(), roots, nr * sizeof(unsigned), hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            rc = MHIT_ERR_HIP;
-        std::sort(hroots.begin(), hroots.end());           // raster order of each drop's first cell
-        if (rc == MHIT_OK && (hipMalloc((void **)&vol, nr * sizeof(unsigned long long)) != hipSuccess)) rc = MHIT_ERR_OOM;
-        std::vector<unsigned long long> hv(nr);
-        if (rc == MHIT_OK &&
-            (hipMemcpyAsync(roots, hroots.data(), nr * sizeof(unsigned), hipMemcpyHostToDevice, st) != hipSuccess ||
-             hipMemsetAsync(vol, 0, nr * sizeof(unsigned long long), st) != hipSuccess))
-            rc = MHIT_ERR_HIP;
-        if (rc == MHIT_OK) {
-            launch_ccl_volume(c->kc, lab, roots, (int)nr, vol, st);
-            if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(hv.data(), vol, nr * sizeof(unsigned long long), hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = MHIT_ERR_HIP;
+    const size_t n3 = (size_t)c->kc.n * c->kc.n * c->kc.n;
+    if (n3 >= (size_t)0xffffffffu) return MHIT_ERR_BAD_N;              // 32-bit labels
+    return MHIT_OK;
+}
+
+// The labelling workspace is allocated by the context's first drop count and
+// kept until the context is destroyed, so that a driver counting drops at
+// every dump does not allocate (and synchronise the device) each time.
+extern "C" int mhit_drop_count(mhit_ctx *c, int max_drops, int *n_drops, double *volume, double *deq)
+{
+    int rc = drop_count_args(c, max_drops, n_drops, volume, deq);
+    if (rc) return rc;
+    if (!c->ccl_ws) {
+        const size_t bytes = ccl_workspace_bytes(c->kc.n);
+        if (hipMalloc(&c->ccl_ws, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            c->ccl_ws = nullptr;
+            return MHIT_ERR_OOM;
         }
-        // deq = dx (6/pi vol)^(1/3), pi = 4 atan(1), dx = lx/(nx-1) (calculate_deq.f90:31; module.f90:3-5)
-        const double n6opi = 6.0 / (4.0 * atan(1.0));
-        for (int q = 0; rc == MHIT_OK && q < (int)nr && q < max_drops; q++) {
-            volume[q] = (double)hv[q];
-            deq[q] = c->cs.dx * pow(n6opi * volume[q], 1.0 / 3.0);
-        }
+        c->ccl_bytes = bytes;
     }
-    if (rc == MHIT_OK) *n_drops = (int)nr;
-    if (lab) (void)hipFree(lab);
-    if (roots) (void)hipFree(roots);
-    if (nroots) (void)hipFree(nroots);
-    if (vol) (void)hipFree(vol);
-    return rc;
+    return drop_count_in(c, ccl_carve(c->ccl_ws, c->kc.n), max_drops, n_drops, volume, deq);
 }
 
 extern "C" int mhit_debug_math(int op, size_t count, const double *x, double *y)
@@ -1305,7 +1334,7 @@ extern "C" int mhit_debug_buffer(mhit_ctx *c, int which, double *host)
 extern "C" int mhit_device_bytes(mhit_ctx *c, size_t *bytes)
 {
     if (!c || !bytes) return MHIT_ERR_ARG;
-    *bytes = c->bytes;
+    *bytes = c->bytes + c->ccl_bytes;
     return MHIT_OK;
 }
 
@@ -1796,6 +1825,10 @@ extern "C" int mhit_ens_submit(mhit_ens *e, const mhit_params *p, const double *
     if (p->nx < 4 || p->nx % 2 != 0 || plan_radices(p->nx, rad) == 0 || p->nx > 2048) return MHIT_ERR_BAD_N;
     if (p->precision != MHIT_FP64 && p->precision != MHIT_MIXED) return MHIT_ERR_UNSUPPORTED;
     if (ctx_bytes(p) > e->cfg.hbm_budget_bytes) return MHIT_ERR_OOM;
+    const int nvel = (u != nullptr) + (v != nullptr) + (w != nullptr);
+    if (nvel != 0 && nvel != 3) return MHIT_ERR_ARG;             // all of u, v, w or none
+    if (p->inflow == 1 && nvel == 0) return MHIT_ERR_ARG;        // inflow=1 asks for given velocity
+    if (p->phiflag && p->inphi == 1 && !phi) return MHIT_ERR_ARG; // inphi=1 asks for a given phi
     auto r = std::make_unique<EnsReq>();
     r->id = e->next_id++;
     r->p = *p;
@@ -1804,9 +1837,9 @@ extern "C" int mhit_ens_submit(mhit_ens *e, const mhit_params *p, const double *
     r->ganged = e->cfg.gang_size > 1 && p->nx <= e->cfg.gang_max_nx;
     const size_t n3 = (size_t)p->nx * p->nx * p->nx;
     const double *src[4] = {u, v, w, phi};
-    if (u && v && w)
+    if (nvel == 3)
         for (int q = 0; q < 3; q++) r->in[q].assign(src[q], src[q] + n3);
-    if (phi) r->in[3].assign(phi, phi + n3);
+    if (phi && p->phiflag) r->in[3].assign(phi, phi + n3);
     r->submit_ms = ens_now(e);
     if (id) *id = r->id;
     e->queue.push_back(r.get());

@@ -9,11 +9,19 @@
  *   orc_derive ............ multi/readinput.f90:35-43      pinned: val t=0 closed form
  *   grid / ktab ........... multi/main.f90:149-161          pinned: val t=0, Poisson KAT
  *   orc_init_ic ........... main.f90:218-233, 263-278       pinned: val t=0 dissipation
- *   phase field A4-A10 .... main.f90:338-486                pinned: numpy literal restatement
- *                                                            (padded halos), sum(phi) invariant,
- *                                                            Fourier-mode closed forms
+ *   phase field A4-A10 .... main.f90:338-486                pinned: ACDI equilibrium of the tanh
+ *                                                            profile (2nd-order residual), Fourier-
+ *                                                            mode eigenvalue and 2nd-order staggered
+ *                                                            convergence of A4, sum(phi) invariant,
+ *                                                            numpy literal restatement
  *   momentum A11-A15 ...... main.f90:513-696                pinned: literal restatement, val
  *                                                            series at t=0.2, closed forms
+ *   ABC forcing A13 ....... main.f90:584-600                pinned: pointwise closed form from rest
+ *   CSF force A14 ......... main.f90:604-658                pinned: zero chemical potential of the
+ *                                                            plane profile (2nd order), Laplace jump
+ *                                                            2 sigma/R and the drop's symmetries
+ *   (tests/oracle_pins.py; tests/test_oracle_mutants.py plants 19 mistakes in
+ *    copies of this file and shows that each is caught)
  *   divergence A16-A17 .... main.f90:715-745                pinned: literal restatement
  *   Poisson A18-A21 ....... main.f90:749-850                pinned: numpy.fft, brute-force DFT,
  *                                                            discrete-mode KAT

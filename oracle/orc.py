@@ -45,16 +45,20 @@ DEFAULTS = dict(nx=16, restart=0, tstart=0, tfin=20000, dump=1000, inflow=0, inp
                 sigma=1.0, epsr=1.0, gamma_factor=1.0, wavenumbers=0, poisson_rhs_fp32=1,
                 precision=0, reserved=0)
 
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
-    lib = ctypes.CDLL(LIB_PATH)
+def load(path=None):
+    """The oracle library at `path` (default: oracle/liboracle.so), loaded
+    once per path.  Other paths are builds of the same source with other
+    flags (the CPU baseline's -O3 -march=native build) or with a planted
+    mistake (the mutation tests of tests/test_oracle_mutants.py)."""
+    path = os.path.abspath(path or LIB_PATH)
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing: run make -C oracle")
+    lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     vp = ctypes.c_void_p
     dp = P(ctypes.c_double)
@@ -87,7 +91,7 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -111,8 +115,8 @@ def consts(**kw):
 class Oracle:
     """One box on the CPU, restating multi/main.f90 at a 1x1 decomposition."""
 
-    def __init__(self, **kw):
-        self.lib = load()
+    def __init__(self, lib_path=None, **kw):
+        self.lib = load(lib_path)
         self.p = params(**kw)
         self.n = self.p.nx
         self.h = self.lib.orc_create(ctypes.byref(self.p))

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from mhit36_amd import Solver, MhitError, MHIT_ERR_CFL, MHIT_ERR_NAN
+from mhit36_amd._lib import MHIT_ERR_ARG, MHIT_ERR_IO
 from oracle import orc
 
 from gpu_util import random_velocity
@@ -86,6 +87,56 @@ def test_restart_resets_to_euler():
         o.step(2)
         for f in (1, 2, 3):
             assert np.array_equal(s.get_field(f), o.get_field(f))
+
+
+def test_restart_with_phase_field_restores_gumax():
+    """A restart (main.f90:318-323) starts with gumax = 1.0, which sets the
+    lagged gamma of the first phase-field step: after reset_time the device
+    steps exactly as a fresh oracle seeded with the same fields."""
+    n = 16
+    kw = dict(nx=n, dt=1e-3, mu=0.01, phiflag=1, sigma=0.2, f1=0.5, f2=0.3, f3=0.2, gamma_factor=1.2)
+    with Solver(**kw) as s:
+        s.init_fields()
+        s.step(3)
+        state = {f: s.get_field(f) for f in (1, 2, 3, 5)}
+        s.reset_time(3)
+        d = s.step(2)
+        assert d.t == 5
+        o = orc.Oracle(**{**kw, "inflow": 1, "inphi": 1})
+        for f, a in state.items():
+            o.set_field(f, a)
+        do = o.step(2)
+        assert d.gumax == do["gumax"]
+        for f in (1, 2, 3, 4, 5):
+            assert np.array_equal(s.get_field(f), o.get_field(f)), f
+
+
+def test_load_state_rejects_other_abi_and_length(tmp_path):
+    n = 16
+    kw = dict(nx=n, dt=1e-3, mu=0.01, phiflag=1)
+    path = str(tmp_path / "s.state")
+    with Solver(**kw) as s:
+        s.init_fields()
+        s.step(2)
+        s.save_state(path)
+        raw = open(path, "rb").read()
+        with Solver(**kw) as t:
+            t.load_state(path)                       # the file as written loads
+            assert np.array_equal(t.get_field(5), s.get_field(5))
+            bad = bytearray(raw)
+            bad[8:12] = (12345).to_bytes(4, "little")    # abi field after the 8-byte magic
+            open(path, "wb").write(bytes(bad))
+            with pytest.raises(MhitError) as e:
+                t.load_state(path)
+            assert e.value.code == MHIT_ERR_ARG
+            open(path, "wb").write(raw[:-8])
+            with pytest.raises(MhitError) as e:
+                t.load_state(path)
+            assert e.value.code == MHIT_ERR_IO
+            open(path, "wb").write(raw + b"\0" * 8)
+            with pytest.raises(MhitError) as e:
+                t.load_state(path)
+            assert e.value.code == MHIT_ERR_IO
 
 
 def test_field_files_follow_readwrite_layout(tmp_path):

@@ -4,7 +4,8 @@ budget admission, and a stop in one box leaves the others untouched."""
 import numpy as np
 import pytest
 
-from mhit36_amd import Ensemble, Solver, default_params, MHIT_ERR_CFL
+from mhit36_amd import Ensemble, Solver, default_params, MhitError, MHIT_ERR_CFL
+from mhit36_amd._lib import MHIT_ERR_ARG
 from mhit36_amd import ic
 
 pytestmark = pytest.mark.gpu
@@ -85,6 +86,25 @@ def test_submitted_fields_are_used():
         rid = ens.submit(default_params(**p), nsteps=5, u=u, v=v, w=w)
         ens.wait_idle()
         assert np.array_equal(ens.get_field(rid, 1), ref)
+
+
+def test_submit_rejects_incomplete_initial_fields():
+    """u, v, w come all three or not at all; inflow = 1 needs them and
+    inphi = 1 (with the phase field on) needs phi: a request is never run on
+    an initial condition other than the one asked for."""
+    n = 16
+    u, v, w = ic.random_velocity(n, seed=4)
+    with Ensemble() as ens:
+        for kw, fields in ((dict(phiflag=0), dict(u=u)),
+                           (dict(phiflag=0), dict(u=u, v=v)),
+                           (dict(phiflag=0, inflow=1), {}),
+                           (dict(phiflag=1, inphi=1), {})):
+            with pytest.raises(MhitError) as e:
+                ens.submit(default_params(nx=n, dt=1e-3, mu=0.01, **kw), nsteps=1, **fields)
+            assert e.value.code == MHIT_ERR_ARG, (kw, sorted(fields))
+        rid = ens.submit(default_params(nx=n, dt=1e-3, mu=0.01, phiflag=0, inflow=1), nsteps=1, u=u, v=v, w=w)
+        ens.wait_idle()
+        assert {r["id"]: r for r in ens.results()}[rid]["status"] == 0
 
 
 def test_close_with_boxes_in_flight():

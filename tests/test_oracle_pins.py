@@ -1,0 +1,15 @@
+"""The oracle's phase field (A4-A10), CSF force (A14) and ABC forcing (A13)
+against what the mathematics fixes (tests/oracle_pins.py): the ACDI
+equilibrium of the tanh profile, the centred-difference eigenvalue of a
+Fourier mode, second-order convergence of the staggered convective term, the
+vanishing chemical potential of a plane interface, Laplace's law for a drop,
+and the ABC forcing pointwise.  tests/test_oracle_mutants.py shows that each
+of these fails when a plausible mistake is planted in the oracle."""
+import pytest
+
+import oracle_pins as P
+
+
+@pytest.mark.parametrize("pin", sorted(P.ALL))
+def test_oracle_pin(pin):
+    P.ALL[pin]()
