@@ -55,28 +55,20 @@ static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box e
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
-// FP64: the project's own table-driven log/tanh (bit-identical to the
-// oracle; `tab` is the block's LDS copy of the table) and IEEE division.
-// FP32 (the mixed-precision pass, compared with FP64 by tolerance): the
-// hardware log2/exp2/sqrt approximations (v_log_f32, v_exp_f32, v_sqrt_f32)
-// and quotients as a product with the hardware reciprocal (v_rcp_f32): a few ulp of FP32, far
-// below the FP32 storage error of the state, in 1-2 instructions where the
-// correctly rounded forms take 8-10.  (HIP's __logf and __fdividef are the
-// correctly rounded forms unless fast math is on.)  The arguments are
-// positive normal numbers or moderate in size: log's is (phi + 1e-16) /
-// (1 - phi + 1e-16) for phi clamped to [0, 1], the divisors 1 + e and
-// |grad psi| + 1e-16.
+// FP64: the project's own table-driven log (bit-identical to the oracle;
+// `tab` is the block's LDS copy of the table) and IEEE division.  FP32 (the
+// mixed-precision pass, compared with FP64 by tolerance): the hardware log2
+// and sqrt approximations (v_log_f32, v_sqrt_f32) and quotients as a product
+// with the hardware reciprocal (v_rcp_f32): a few ulp of FP32, far below the
+// FP32 storage error of the state, in 1-2 instructions where the correctly
+// rounded forms take 8-10.  (HIP's __logf and __fdividef are the correctly
+// rounded forms unless fast math is on.)  The arguments are positive normal
+// numbers or moderate in size: log's is (phi + 1e-16) / (1 - phi + 1e-16)
+// for phi clamped to [0, 1], the divisor |grad psi| + 1e-16.
 __device__ __forceinline__ double tlog(double x, const double *tab) { return dlog(x, tab); }
 __device__ __forceinline__ float tlog(float x, const double *)
 {
     return __builtin_amdgcn_logf(x) * 0.6931471805599453f;       // log2(x) ln 2
-}
-__device__ __forceinline__ double ttanh(double x, const double *tab) { return dtanh(x, tab); }
-__device__ __forceinline__ float ttanh(float x, const double *)
-{
-    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * __builtin_fabsf(x));   // exp(-2|x|)
-    const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);                       // tanh|x|
-    return x < 0.0f ? -t : t;
 }
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -99,7 +91,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const int n = c.n;
     const int tid = threadIdx.x;
     const int i0 = blk.x * ATX, j0 = blk.y * ATY, k0 = blk.z * kz;
-    const T eps = c.eps, epsi = c.epsi, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
+    const T eps = c.eps, en = c.enumr, dxi = c.dxi, ddxi = c.ddxi;
     const T um = sc->umean, vm = sc->vmean, wm = sc->wmean;
     const T gamma = c.gamma_factor * sc->gumax;
     const T tiny = (T)1.0E-16f;
@@ -143,9 +135,14 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
         for (int q = 0; q < 2; q++)
             if (pin[q] >= 0) phL[slot][pin[q]] = pre[q];
     };
+    // tanh(psi/(2 eps)) of A9 through the identity tanh(log(q)/2) = (q-1)/(q+1)
+    // = (2 val - 1)/(1 + 2 enum) for psi = eps log q, q = (val + enum)/(1 - val +
+    // enum): from the cell's phi, no tanh.  A NaN psi (phi below -enum, Q7)
+    // stays NaN, as tanh(NaN) would.
+    const T sharp_c = (T)(1.0 / (1.0 + 2.0 * c.enumr));
     auto s_plane = [&](int m) {              // A7 + A8 + a(psi) on plane m
         const int ps = (m - k0 + 3) % 3, psm = (m - 1 - k0 + 3) % 3, psp = (m + 1 - k0 + 3) % 3;
-        const int sxy = (m - k0 + 2) & 1, sz = (m - k0 + 3) % 3;
+        const int sxy = (m - k0 + 2) & 1, sz = (m - k0 + 3) % 3, phs = (m - k0 + 4) & 3;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             int e = tid + q * ATHREADS;
@@ -156,7 +153,10 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
                 T gz = psiL[psp][pc] - psiL[psm][pc];
                 T normod = tdiv((T)1.0, tsqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                T th = ttanh(((T)0.5 * psiL[ps][pc]) * epsi, tabL);
+                const T f = phL[phs][e], pc0 = psiL[ps][pc];
+                T val = f < (T)1.0 ? f : (T)1.0;
+                if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // as psi's FP32 clamp
+                T th = pc0 != pc0 ? pc0 : ((T)2.0 * val - (T)1.0) * sharp_c;
                 T a = (T)0.25 * ((T)1.0 - th * th);
                 sxyL[sxy][0][e] = a * (gx * normod);
                 sxyL[sxy][1][e] = a * (gy * normod);

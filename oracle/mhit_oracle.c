@@ -409,6 +409,7 @@ struct orc_state {
     double *phi, *rphi, *psi, *nxv, *nyv, *nzv, *sx, *sy, *sz, *fx, *fy, *fz;
     double *rhsp, *pr, *spec;
     fftplan plan;
+    int sharpening;            /* 0: tanh of psi as written; 1: the identity form (orc_set_sharpening) */
     int nsteps_since_init;     /* 0 => next step is Euler (main.f90:323-324) */
     int t;
     double gumax;
@@ -495,6 +496,8 @@ void orc_destroy(orc_state *s)
 }
 
 void orc_get_consts(const orc_state *s, orc_consts *c) { *c = s->c; }
+
+void orc_set_sharpening(orc_state *s, int form) { s->sharpening = form ? 1 : 0; }
 
 void orc_get_tables(const orc_state *s, double *x, double *ktab)
 {
@@ -757,6 +760,7 @@ static void phase_field(orc_state *s, double gamma)
      * the sharpening flux a*n (a = 0.25*(1 - tanh^2(psi/(2 eps)))) is formed
      * per cell once and used by both neighbours (same value either way). */
     const double tiny = (double)1.0E-16f;
+    const double sharp_c = 1.0 / (1.0 + 2.0 * en);
     #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
         int kp = (k + 1) % n, km = (k + n - 1) % n;
@@ -768,7 +772,18 @@ static void phase_field(orc_state *s, double gamma)
                 double gy = psi[IDX(i, jp, k, n)] - psi[IDX(i, jm, k, n)];
                 double gz = psi[IDX(i, j, kp, n)] - psi[IDX(i, j, km, n)];
                 double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                double th = orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi);
+                double th;
+                if (s->sharpening) {
+                    /* tanh(psi/(2 eps)) = tanh(log(q)/2) = (q-1)/(q+1) = (2 val - 1)/(1 + 2 enum)
+                     * for psi = eps log q, q = (val + enum)/(1 - val + enum); a NaN psi
+                     * (phi below -enum, Q7) stays NaN as tanh(NaN) does. */
+                    double f = phi[IDX(i, j, k, n)];
+                    double val = f < 1.0 ? f : 1.0;
+                    double ps = psi[IDX(i, j, k, n)];
+                    th = (ps != ps) ? ps : (2.0 * val - 1.0) * sharp_c;
+                } else {
+                    th = orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi);
+                }
                 double a = 0.25 * (1.0 - th * th);
                 size_t id = IDX(i, j, k, n);
                 s->sx[id] = a * (gx * normod);

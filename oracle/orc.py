@@ -73,6 +73,7 @@ def load(path=None):
         "orc_init_ic": ([vp], None),
         "orc_step": ([vp, ctypes.c_int, P(OrcDiag)], ctypes.c_int),
         "orc_get_consts": ([vp, P(OrcConsts)], None),
+        "orc_set_sharpening": ([vp, ctypes.c_int], None),
         "orc_get_tables": ([vp, dp, dp], None),
         "orc_debug_buffer": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "orc_fft_plan_radices": ([ctypes.c_int, P(ctypes.c_int)], ctypes.c_int),
@@ -112,16 +113,25 @@ def consts(**kw):
     return {name: getattr(c, name) for name, _ in OrcConsts._fields_}
 
 
-class Oracle:
-    """One box on the CPU, restating multi/main.f90 at a 1x1 decomposition."""
+SHARPENING = {"literal": 0, "identity": 1}
 
-    def __init__(self, lib_path=None, **kw):
+
+class Oracle:
+    """One box on the CPU, restating multi/main.f90 at a 1x1 decomposition.
+
+    sharpening: how the ACDI sharpening factor 1/4 (1 - tanh^2(psi/(2 eps)))
+    is evaluated -- "identity" (default: the form the HIP path evaluates,
+    (2 val - 1)/(1 + 2 enum) for tanh(psi/(2 eps))) or "literal" (tanh of
+    psi as main.f90:463-465 writes it).  They differ by rounding only."""
+
+    def __init__(self, lib_path=None, sharpening="identity", **kw):
         self.lib = load(lib_path)
         self.p = params(**kw)
         self.n = self.p.nx
         self.h = self.lib.orc_create(ctypes.byref(self.p))
         if not self.h:
             raise ValueError(f"oracle rejects nx={self.n}")
+        self.lib.orc_set_sharpening(self.h, SHARPENING[sharpening])
 
     def close(self):
         if self.h:

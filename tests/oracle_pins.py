@@ -2,8 +2,10 @@
 ABC forcing against what the mathematics fixes, independently of how the
 oracle writes the reference's expressions.
 
-Each function runs the oracle built at `lib` (None: oracle/liboracle.so) and
-raises AssertionError when its check fails.  tests/test_oracle_pins.py runs
+Each function runs the oracle built at `lib` (None: oracle/liboracle.so),
+with the sharpening factor in `form` ("identity", the form the HIP path
+evaluates, or "literal", tanh of psi as written), and raises AssertionError
+when its check fails.  tests/test_oracle_pins.py runs
 them on the oracle; tests/test_oracle_mutants.py runs them on copies of the
 oracle with one planted mistake each and requires every mistake to be caught.
 
@@ -21,8 +23,8 @@ import numpy as np
 from oracle import orc
 
 
-def _box(lib, n, **kw):
-    o = orc.Oracle(lib_path=lib, nx=n, inflow=1, inphi=1, **kw)
+def _box(lib, n, form="identity", **kw):
+    o = orc.Oracle(lib_path=lib, sharpening=form, nx=n, inflow=1, inphi=1, **kw)
     c = o.consts()
     x = np.arange(n) * c["dx"]
     return o, c, x
@@ -53,10 +55,10 @@ def _lap(f, dx):
 # its argument, a one-sided normal or a missing eps leaves an O(1) or
 # first-order residual.
 # ---------------------------------------------------------------------------
-def sphere_residuals(lib=None, n=128, R=2.0, epsrs=(1.0, 2.0, 4.0)):
+def sphere_residuals(lib=None, n=128, R=2.0, epsrs=(1.0, 2.0, 4.0), form="identity"):
     out = []
     for er in epsrs:
-        o, c, x = _box(lib, n, dt=1.0, mu=0.0, phiflag=1, sigma=0.0, epsr=er)
+        o, c, x = _box(lib, n, form, dt=1.0, mu=0.0, phiflag=1, sigma=0.0, epsr=er)
         eps, dx = c["eps"], c["dx"]
         X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
         cx, cy, cz = x[n // 2] + 0.37 * dx, x[n // 2] - 0.21 * dx, x[n // 2] + 0.11 * dx
@@ -70,8 +72,8 @@ def sphere_residuals(lib=None, n=128, R=2.0, epsrs=(1.0, 2.0, 4.0)):
     return out
 
 
-def check_sphere_equilibrium(lib=None):
-    r1, r2, r4 = sphere_residuals(lib)
+def check_sphere_equilibrium(lib=None, form="identity"):
+    r1, r2, r4 = sphere_residuals(lib, form=form)
     assert r1 < 0.12, (r1, r2, r4)
     assert 3.3 < r1 / r2 < 4.8, (r1, r2, r4)
     assert 3.3 < r2 / r4 < 5.0, (r1, r2, r4)
@@ -84,9 +86,9 @@ def check_sphere_equilibrium(lib=None):
 #   rhs_i = U c1 sin(theta) sin(theta i + s) / dx,
 # the eigenvalue of the centred difference on that mode.
 # ---------------------------------------------------------------------------
-def check_convection_mode(lib=None, n=32, m=3, U=0.7, dt=1e-2):
+def check_convection_mode(lib=None, form="identity", n=32, m=3, U=0.7, dt=1e-2):
     for axis in range(3):
-        o, c, x = _box(lib, n, dt=dt, mu=0.0, phiflag=1, sigma=0.0, gamma_factor=0.0)
+        o, c, x = _box(lib, n, form, dt=dt, mu=0.0, phiflag=1, sigma=0.0, gamma_factor=0.0)
         dx = c["dx"]
         th = 2 * np.pi * m / n
         i = np.arange(n)
@@ -110,10 +112,10 @@ def check_convection_mode(lib=None, n=32, m=3, U=0.7, dt=1e-2):
 # (error ratio ~4 per doubling of N); velocities read from the wrong faces
 # shift u by dx and leave a first-order error.
 # ---------------------------------------------------------------------------
-def convection_errors(lib=None, axis=0, ns=(32, 64, 128), dt=1e-2):
+def convection_errors(lib=None, axis=0, ns=(32, 64, 128), dt=1e-2, form="identity"):
     errs = []
     for n in ns:
-        o, c, x = _box(lib, n, dt=dt, mu=0.0, phiflag=1, sigma=0.0, gamma_factor=0.0)
+        o, c, x = _box(lib, n, form, dt=dt, mu=0.0, phiflag=1, sigma=0.0, gamma_factor=0.0)
         dx = c["dx"]
         k = 2 * np.pi / (n * dx)                      # one period over the n-point grid
         xc = np.arange(n) * dx
@@ -134,9 +136,9 @@ def convection_errors(lib=None, axis=0, ns=(32, 64, 128), dt=1e-2):
     return errs
 
 
-def check_convection_staggering(lib=None):
+def check_convection_staggering(lib=None, form="identity"):
     for axis in range(3):
-        e = convection_errors(lib, axis)
+        e = convection_errors(lib, axis, form=form)
         assert e[0] < 0.02, (axis, e)
         assert 3.5 < e[0] / e[1] < 4.5 and 3.5 < e[1] / e[2] < 4.5, (axis, e)
 
@@ -150,10 +152,10 @@ def check_convection_staggering(lib=None):
 # error in either term, or a wrong factor, leaves it O(1).  The force is
 # checked in each axis, so a wrong neighbour in one direction shows too.
 # ---------------------------------------------------------------------------
-def csf_plane_residuals(lib=None, axis=0, n=128, epsrs=(1.0, 2.0, 4.0), R=1.5):
+def csf_plane_residuals(lib=None, axis=0, n=128, epsrs=(1.0, 2.0, 4.0), R=1.5, form="identity"):
     out = []
     for er in epsrs:
-        o, c, x = _box(lib, n, dt=1e-3, mu=0.0, phiflag=1, sigma=0.3, epsr=er, gamma_factor=0.0)
+        o, c, x = _box(lib, n, form, dt=1e-3, mu=0.0, phiflag=1, sigma=0.3, epsr=er, gamma_factor=0.0)
         eps, dx = c["eps"], c["dx"]
         p1 = 0.5 * (1.0 - np.tanh((np.abs(x - x[n // 2] - 0.3 * dx) - R) / (2.0 * eps)))
         sh = [1, 1, 1]
@@ -171,9 +173,9 @@ def csf_plane_residuals(lib=None, axis=0, n=128, epsrs=(1.0, 2.0, 4.0), R=1.5):
     return out
 
 
-def check_csf_plane(lib=None):
+def check_csf_plane(lib=None, form="identity"):
     for axis in range(3):
-        r1, r2, r4 = csf_plane_residuals(lib, axis)
+        r1, r2, r4 = csf_plane_residuals(lib, axis, form=form)
         assert r1 < 0.3, (axis, r1, r2, r4)
         assert 3.3 < r1 / r2 < 4.8 and 3.3 < r2 / r4 < 4.8, (axis, r1, r2, r4)
 
@@ -187,8 +189,8 @@ def check_csf_plane(lib=None):
 # reflection about the centre and under axis permutations: forces
 # interpolated to the wrong faces break that.
 # ---------------------------------------------------------------------------
-def laplace_jump(lib=None, n=96, R=1.0, epsr=1.0, sigma=0.1):
-    o, c, x = _box(lib, n, dt=1e-3, mu=0.0, phiflag=1, sigma=sigma, epsr=epsr, gamma_factor=0.0,
+def laplace_jump(lib=None, n=96, R=1.0, epsr=1.0, sigma=0.1, form="identity"):
+    o, c, x = _box(lib, n, form, dt=1e-3, mu=0.0, phiflag=1, sigma=sigma, epsr=epsr, gamma_factor=0.0,
                    wavenumbers=1, poisson_rhs_fp32=0)
     eps = c["eps"]
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
@@ -207,9 +209,9 @@ def laplace_jump(lib=None, n=96, R=1.0, epsr=1.0, sigma=0.1):
     return jump, asym
 
 
-def check_laplace(lib=None):
-    j1, a1 = laplace_jump(lib, epsr=1.0)
-    j2, a2 = laplace_jump(lib, epsr=2.0)
+def check_laplace(lib=None, form="identity"):
+    j1, a1 = laplace_jump(lib, epsr=1.0, form=form)
+    j2, a2 = laplace_jump(lib, epsr=2.0, form=form)
     assert abs(j1 - 1.0) < 0.05, j1
     assert abs(j2 - 1.0) < 0.02, j2
     assert a1 < 1e-11 and a2 < 1e-11, (a1, a2)
@@ -223,9 +225,9 @@ def check_laplace(lib=None):
 #   v* = dt (f1 sin(k0 x_i) + f3 cos(k0 z_k))
 #   w* = dt (f2 sin(k0 y_j) + f1 cos(k0 x_i)).
 # ---------------------------------------------------------------------------
-def check_forcing(lib=None, n=16, dt=1e-2, k0=2.0):
+def check_forcing(lib=None, form="identity", n=16, dt=1e-2, k0=2.0):
     f1, f2, f3 = 0.3, 0.5, 0.7
-    o, c, _ = _box(lib, n, dt=dt, mu=0.01, f1=f1, f2=f2, f3=f3, k0=k0)
+    o, c, _ = _box(lib, n, form, dt=dt, mu=0.01, f1=f1, f2=f2, f3=f3, k0=k0)
     x = np.arange(n) * c["dx"]
     _set(o, n)
     o.step(1)

@@ -50,3 +50,29 @@ def test_wrapping_and_single_cells():
     rc, rvol, rdeq = orc.drop_count(used)
     assert cnt == rc == 65
     assert np.array_equal(vol, rvol) and np.array_equal(deq, rdeq)
+
+
+def test_as_many_drops_as_the_box_holds():
+    """Isolated cells two apart in every direction: n^3/8 drops, the most a
+    box can hold, spread over every chunk of the device's root listing."""
+    n = 32
+    phi = np.zeros((n, n, n))
+    phi[::2, ::2, ::2] = 1.0
+    cnt, vol, deq, used = device_count(n, phi)
+    rc, rvol, rdeq = orc.drop_count(used)
+    assert cnt == rc == n ** 3 // 8
+    assert np.array_equal(vol, rvol) and np.array_equal(deq, rdeq)
+
+
+def test_workspace_reused_across_calls():
+    """A context keeps its labelling workspace: a second count on another
+    phi sees nothing of the first."""
+    n = 48
+    a = np.random.default_rng(3).uniform(0, 1, (n, n, n)) ** 4
+    b = np.random.default_rng(4).uniform(0, 1, (n, n, n)) ** 9
+    with Solver(nx=n, dt=1e-3, mu=0.01, phiflag=1, inphi=1) as s:
+        for phi in (a, b, a):
+            s.set_field(5, phi)
+            cnt, vol, deq = s.drop_count()
+            rc, rvol, rdeq = orc.drop_count(s.get_field(5))
+            assert cnt == rc and np.array_equal(vol, rvol) and np.array_equal(deq, rdeq)

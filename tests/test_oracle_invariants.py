@@ -143,3 +143,35 @@ def test_euler_then_ab2():
         c.set_field(f, b.get_field(f))
     c.step(1)                           # Euler from the state after step 1
     assert not np.array_equal(c.get_field(1), a.get_field(1))   # AB2 differs from Euler
+
+
+@pytest.mark.parametrize("fp32_rhs", [0, 1])
+def test_sharpening_identity_matches_literal_tanh(fp32_rhs):
+    """The HIP path evaluates tanh(psi/(2 eps)) through the identity
+    (2 val - 1)/(1 + 2 enum); the oracle has both forms, which differ by
+    rounding only.  Over 200 steps of a forced flow with drops:
+    - with an FP64 Poisson right-hand side the fields stay within 1e-13;
+    - with the reference's REAL(4) right-hand side (Q3, main.f90:738) one
+      rounding difference can flip a float rounding of rhsp, which moves p by
+      an FP32 ulp; u, v, w, phi then stay within 1e-10 (the FP64 tolerance
+      of the target) and p within 1e-7."""
+    n = 32
+    u, v, w = random_velocity(n, seed=21, amp=0.7)
+    dx = 2 * np.pi / (n - 1)
+    phi = drops_phi(n, [(2.0, 2.5, 3.0), (4.5, 4.0, 2.0)], 0.8, dx)
+    kw = dict(nx=n, dt=2e-3, mu=0.01, phiflag=1, inflow=1, inphi=1, sigma=0.1, gamma_factor=1.2,
+              f1=0.3, f2=0.2, f3=0.1, poisson_rhs_fp32=fp32_rhs)
+    out = {}
+    for form in ("identity", "literal"):
+        o = orc.Oracle(sharpening=form, **kw)
+        for f, a in ((1, u), (2, v), (3, w), (5, phi)):
+            o.set_field(f, a)
+        d = o.step(200)
+        assert d["status"] == 0
+        out[form] = [o.get_field(f) for f in (1, 2, 3, 5, 4)]
+    diffs = [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(out["identity"], out["literal"])]
+    assert max(diffs) > 0.0                  # the two forms do round differently
+    if fp32_rhs:
+        assert max(diffs[:4]) < 1e-10 and diffs[4] < 1e-7, diffs
+    else:
+        assert max(diffs) < 1e-13, diffs

@@ -19,21 +19,25 @@ import oracle_pins as P
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "mhit_oracle.c")
 
-# (name, exact text in the oracle, planted replacement, pin that must fail)
+# (name, exact text in the oracle, planted replacement, pin that must fail
+# [, sharpening form the pin runs with; default the identity form])
 MUTANTS = [
     ("sharpening_sign", "double r = rphi[id] - gamma * ((X + Y) + Z);",
      "double r = rphi[id] + gamma * ((X + Y) + Z);", "sphere_equilibrium"),
     ("tanh_factor", "double a = 0.25 * (1.0 - th * th);",
      "double a = 0.5 * (1.0 - th * th);", "sphere_equilibrium"),
     ("tanh_argument", "orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi)",
-     "orc_tanh(psi[IDX(i, j, k, n)] * epsi)", "sphere_equilibrium"),
+     "orc_tanh(psi[IDX(i, j, k, n)] * epsi)", "sphere_equilibrium", "literal"),
+    ("identity_offset", "(2.0 * val - 1.0) * sharp_c", "(2.0 * val - 0.5) * sharp_c", "sphere_equilibrium"),
     ("diffusion_y_no_eps", "double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;",
      "double dyy = ((fyp - 2.0 * f0) + fym) * ddxi;", "sphere_equilibrium"),
     ("normal_y_one_sided", "double gy = psi[IDX(i, jp, k, n)] - psi[IDX(i, jm, k, n)];",
      "double gy = 2.0 * (psi[IDX(i, jp, k, n)] - psi[IDX(i, j, k, n)]);", "sphere_equilibrium"),
     ("normal_no_sqrt", "double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);",
      "double normod = 1.0 / (((gx * gx + gy * gy) + gz * gz) + tiny);", "sphere_equilibrium"),
-    ("psi_no_eps", "psi[q] = eps * orc_log(", "psi[q] = orc_log(", "sphere_equilibrium"),
+    # psi enters the identity form only through its normalised gradient, so a
+    # missing eps changes only the literal form's tanh argument
+    ("psi_no_eps", "psi[q] = eps * orc_log(", "psi[q] = orc_log(", "sphere_equilibrium", "literal"),
     ("convection_x_staggering",
      "double A = u[IDX(ip, j, k, n)] * 0.5 * (fxp + f0) - u[IDX(i, j, k, n)] * 0.5 * (f0 + fxm);",
      "double A = u[IDX(i, j, k, n)] * 0.5 * (fxp + f0) - u[IDX(im, j, k, n)] * 0.5 * (f0 + fxm);",
@@ -82,7 +86,7 @@ def mutant_libs(tmp_path_factory):
     flags = _cflags()
 
     def build(m):
-        name, old, new, _ = m
+        name, old, new = m[:3]
         assert src.count(old) == 1, f"{name}: site not found exactly once in the oracle"
         c = base / f"{name}.c"
         c.write_text(src.replace(old, new))
@@ -96,10 +100,12 @@ def mutant_libs(tmp_path_factory):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("name,old,new,pin", MUTANTS, ids=[m[0] for m in MUTANTS])
-def test_pin_catches_planted_mistake(mutant_libs, name, old, new, pin):
+@pytest.mark.parametrize("m", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_pin_catches_planted_mistake(mutant_libs, m):
+    name, pin = m[0], m[3]
+    form = m[4] if len(m) > 4 else "identity"
     with pytest.raises(AssertionError):
-        P.ALL[pin](mutant_libs[name])
+        P.ALL[pin](mutant_libs[name], form=form)
 
 
 @pytest.mark.parametrize("pin", sorted(P.ALL))
