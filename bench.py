@@ -1,7 +1,7 @@
 """Benchmark: ms per timestep of the FP64 512^3 NS+ACDI MHIT36 step on one MI355X.
 
 Workload (the 512^3 NS+ACDI box): random-phase divergence-free velocity,
-E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (seed 1234); ABC forcing A=B=C=1 at
+E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (seed 1234, made on the device); ABC forcing A=B=C=1 at
 k0=2; nu=0.001, rho=1; 512 non-overlapping drops of radius 0.1 (seed 4321),
 eps = dx, gamma = 1.2 umax, sigma = 0.1; dt = 0.1 dx / umax0 (fixed; the
 explicit phase-field update is unstable above CFL ~0.14).  Inputs
@@ -48,8 +48,12 @@ def measured_traffic(n, phiflag, kernel):
 
 
 def make_inputs(n, phiflag, seed_u=1234, seed_phi=4321, ndrops=512, radius=0.1):
-    from mhit36_amd import ic
-    u, v, w = ic.random_velocity(n, seed=seed_u, k_peak=4.0, urms=1.0)
+    """Host copies of the workload's initial fields: the random-phase velocity
+    made on the device (mhit_init_random_velocity) and the drops (ic.drops)."""
+    from mhit36_amd import Solver, ic
+    with Solver(nx=n, dt=1e-3, mu=0.001, phiflag=0, inflow=1) as s:
+        s.init_random_velocity(seed=seed_u, k_peak=4.0, urms=1.0)
+        u, v, w = (s.get_field(f) for f in (1, 2, 3))
     _, dx = ic.grid(n)
     phi = None
     if phiflag:

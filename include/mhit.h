@@ -150,6 +150,19 @@ int mhit_step(mhit_ctx *ctx, int nsteps, mhit_diag *diag);
 #define MHIT_BATCH_MAX 16
 int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_diag *diag);
 
+/* Random-phase turbulent initial velocity on the device (no counterpart in
+ * the reference, which reads such a field from input/{u,v,w}.dat with
+ * inflow = 1, main.f90:234-239): u = curl A of a random vector potential,
+ * A(k) = amp(k) g e^{i theta} on the half spectrum with amp(k) =
+ * sqrt(exp(-2 (k/k_peak)^2) / (4 pi)) for 0 < |k| <= min(N/2-1, 6 k_peak), g
+ * standard normal and theta uniform from a SplitMix64 hash of (seed,
+ * component, position); the curl uses the staggered grid's forward
+ * differences, so the discrete divergence of main.f90:723-725 vanishes up to
+ * rounding, and E(k) ~ k^4 exp(-2 (k/k_peak)^2).  Scaled to
+ * urms = sqrt(<u^2+v^2+w^2>/3).  Overwrites u, v, w (and the spectrum
+ * buffer); needs 24 N^3 bytes of device memory for the duration of the call. */
+int mhit_init_random_velocity(mhit_ctx *ctx, unsigned long long seed, double k_peak, double urms);
+
 /* Restart (main.f90:318-323): set the step counter, make the next step an
  * Euler step and set gumax to 1.0, the value behind the lagged gamma of a
  * run's first step (main.f90:321, 362).  Clears a CFL/NaN halt. */

@@ -30,7 +30,7 @@ MHIT_BATCH_MAX = 16
 EXPORTS = (
     "mhit_default_params", "mhit_read_input", "mhit_derive", "mhit_create",
     "mhit_destroy", "mhit_init_fields", "mhit_set_field", "mhit_get_field",
-    "mhit_step", "mhit_step_batch", "mhit_reset_time", "mhit_write_field", "mhit_read_field",
+    "mhit_step", "mhit_step_batch", "mhit_reset_time", "mhit_init_random_velocity", "mhit_write_field", "mhit_read_field",
     "mhit_diagnostics", "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
     "mhit_debug_math", "mhit_debug_buffer", "mhit_autotune", "mhit_get_tuning", "mhit_set_tuning", "mhit_drop_count", "mhit_save_state", "mhit_load_state",
     "mhit_ens_create", "mhit_ens_destroy", "mhit_ens_submit", "mhit_ens_progress", "mhit_ens_wait_idle",
@@ -130,6 +130,7 @@ def load():
         "mhit_step": ([vp, ctypes.c_int, P(Diag)], ctypes.c_int),
         "mhit_step_batch": ([P(vp), ctypes.c_int, ctypes.c_int, P(Diag)], ctypes.c_int),
         "mhit_reset_time": ([vp, ctypes.c_int], ctypes.c_int),
+        "mhit_init_random_velocity": ([vp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "mhit_write_field": ([vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mhit_read_field": ([vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
         "mhit_diagnostics": ([vp, ctypes.c_double, ctypes.c_double, P(Stats)], ctypes.c_int),

@@ -163,6 +163,13 @@ void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream
 // Label, list the roots in raster order, count each drop's cells.  Afterwards
 // *w.nroots holds the drop count; w.vol[0..count) the volumes when count <= cap.
 void launch_ccl_count(const Fields &f, const Consts &c, const CclWork &w, hipStream_t s);
+// Random-phase initial velocity: the potential A_comp on the grid (into `out`,
+// n^3 FP64; uses the spectrum buffer), then the staggered curl (sum: per-line
+// sums of |u|^2 into lsum; otherwise u, v, w = scale * curl A, stored as T).
+void launch_ic_potential(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, int comp,
+                         unsigned long long seedmix, double kp, double kmax, double *out, hipStream_t s);
+void launch_ic_curl(const Consts &c, const double *ax, const double *ay, const double *az, void *u, void *v,
+                    void *w, double *lsum, double dx, double scale, bool sum, hipStream_t s);
 // The log/expm1 table of devmath.h, MATH_TAB doubles (host).
 void build_math_table(double *out);
 void launch_diagnostics(const Fields &f, const Consts &c, double dxi, double *out, hipStream_t s);

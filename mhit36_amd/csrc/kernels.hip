@@ -135,11 +135,11 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
         for (int q = 0; q < 2; q++)
             if (pin[q] >= 0) phL[slot][pin[q]] = pre[q];
     };
-    // tanh(psi/(2 eps)) of A9 through the identity tanh(log(q)/2) = (q-1)/(q+1)
-    // = (2 val - 1)/(1 + 2 enum) for psi = eps log q, q = (val + enum)/(1 - val +
-    // enum): from the cell's phi, no tanh.  A NaN psi (phi below -enum, Q7)
-    // stays NaN, as tanh(NaN) would.
-    const T sharp_c = (T)(1.0 / (1.0 + 2.0 * c.enumr));
+    // The factor 1/4 (1 - tanh^2(psi/(2 eps))) of A9 from the cell's phi, no
+    // tanh: psi = eps log(A/B), A = val + enum, B = (1 - val) + enum, so it is
+    // A B / (A + B)^2 exactly (and, like the tanh form, goes to 0 with A as phi
+    // nears -enum).  A NaN psi (phi below -enum, Q7) stays NaN, as tanh(NaN)
+    // would.
     auto s_plane = [&](int m) {              // A7 + A8 + a(psi) on plane m
         const int ps = (m - k0 + 3) % 3, psm = (m - 1 - k0 + 3) % 3, psp = (m + 1 - k0 + 3) % 3;
         const int sxy = (m - k0 + 2) & 1, sz = (m - k0 + 3) % 3, phs = (m - k0 + 4) & 3;
@@ -156,8 +156,8 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 const T f = phL[phs][e], pc0 = psiL[ps][pc];
                 T val = f < (T)1.0 ? f : (T)1.0;
                 if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // as psi's FP32 clamp
-                T th = pc0 != pc0 ? pc0 : ((T)2.0 * val - (T)1.0) * sharp_c;
-                T a = (T)0.25 * ((T)1.0 - th * th);
+                const T A = val + en, B = ((T)1.0 - val) + en, S = A + B;
+                const T a = pc0 != pc0 ? pc0 : tdiv(A * B, S * S);
                 sxyL[sxy][0][e] = a * (gx * normod);
                 sxyL[sxy][1][e] = a * (gy * normod);
                 szL[sz][e] = a * (gz * normod);
@@ -950,6 +950,9 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__
 
 // Column pass along z: forward FFT, division by -k^2 and N^3 with the mean
 // mode set to zero (A19, main.f90:765-798), inverse FFT.  Block = (kx tile, j).
+// INV_ONLY: the inverse z transform alone (no forward transform, no divide),
+// for the initial-condition generator.
+template <bool INV_ONLY = false>
 __device__ __forceinline__ void k_zpoisson_body(uint3 blk, double2 *__restrict__ spec,
                                                               const double2 *__restrict__ tw,
                                                               const double *__restrict__ ktab,
@@ -968,22 +971,24 @@ __device__ __forceinline__ void k_zpoisson_body(uint3 blk, double2 *__restrict__
         lds[r * tkx + ql] = ql < nq ? base[(size_t)r * rs + ql] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<-1>(lds, tkx, 1, tkx, pl, tw);
-    const double nnn = (double)((long long)n * (long long)n * (long long)n);
-    const double ky = ktab[j];
-    for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
-        int r = e / tkx, ql = e - r * tkx;
-        int q = q0 + ql;
-        if (ql < nq) {
-            double kx = ktab[q], kz = ktab[r];
-            double k2 = (kx * kx + ky * ky) + kz * kz;
-            double2 z = lds[r * tkx + ql];
-            z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
-            if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
-            lds[r * tkx + ql] = z;
+    if constexpr (!INV_ONLY) {
+        fft_lds<-1>(lds, tkx, 1, tkx, pl, tw);
+        const double nnn = (double)((long long)n * (long long)n * (long long)n);
+        const double ky = ktab[j];
+        for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
+            int r = e / tkx, ql = e - r * tkx;
+            int q = q0 + ql;
+            if (ql < nq) {
+                double kx = ktab[q], kz = ktab[r];
+                double k2 = (kx * kx + ky * ky) + kz * kz;
+                double2 z = lds[r * tkx + ql];
+                z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
+                if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
+                lds[r * tkx + ql] = z;
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
     fft_lds<1>(lds, tkx, 1, tkx, pl, tw);
     for (int e = threadIdx.x; e < n * tkx; e += blockDim.x) {
         int r = e / tkx, ql = e - r * tkx;
@@ -997,7 +1002,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restric
                                                               const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                               int tkx)
 {
-    k_zpoisson_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+    k_zpoisson_body<false>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
 }
 
 
@@ -1240,6 +1245,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict
 }
 
 
+template <bool INV_ONLY = false>
 __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict__ spec,
                                                                 const double2 *__restrict__ tw,
                                                                 const double *__restrict__ ktab,
@@ -1258,18 +1264,20 @@ __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict
     double2 v[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * rs] : make_double2(0.0, 0.0);
-    fft_regs<-1, false>(v, tj, line, lds, 1, tkx, pl, tw);
-    const double nnn = (double)((long long)n * (long long)n * (long long)n);
-    const double ky = ktab[j], kx = ktab[ok ? q : 0];
+    if constexpr (!INV_ONLY) {
+        fft_regs<-1, false>(v, tj, line, lds, 1, tkx, pl, tw);
+        const double nnn = (double)((long long)n * (long long)n * (long long)n);
+        const double ky = ktab[j], kx = ktab[ok ? q : 0];
 #pragma unroll
-    for (int m = 0; m < 8; m++) {
-        const int r = tj + m * LM;
-        double kz = ktab[r];
-        double k2 = (kx * kx + ky * ky) + kz * kz;
-        double2 z = v[m];
-        z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
-        if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
-        v[m] = z;
+        for (int m = 0; m < 8; m++) {
+            const int r = tj + m * LM;
+            double kz = ktab[r];
+            double k2 = (kx * kx + ky * ky) + kz * kz;
+            double2 z = v[m];
+            z = make_double2(((-z.x) / k2) / nnn, ((-z.y) / k2) / nnn);
+            if (q == 0 && j == 0 && r == 0) z = make_double2(0.0, 0.0);
+            v[m] = z;
+        }
     }
     fft_regs<1, false>(v, tj, line, lds, 1, tkx, pl, tw);
     if (ok) {
@@ -1278,13 +1286,31 @@ __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict
     }
 }
 
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zinv_r(double2 *__restrict__ spec,
+                                                            const double2 *__restrict__ tw,
+                                                            const double *__restrict__ ktab,
+                                                            const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                            int tkx)
+{
+    k_zpoisson_r_body<true>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+}
+
+__global__ void __launch_bounds__(FFT_MAX_THREADS) k_zinv(double2 *__restrict__ spec,
+                                                          const double2 *__restrict__ tw,
+                                                          const double *__restrict__ ktab,
+                                                          const Scalars *__restrict__ sc, FftPlan pl, int ld,
+                                                          int tkx)
+{
+    k_zpoisson_body<true>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+}
+
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restrict__ spec,
                                                                 const double2 *__restrict__ tw,
                                                                 const double *__restrict__ ktab,
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                                 int tkx)
 {
-    k_zpoisson_r_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+    k_zpoisson_r_body<false>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
 }
 
 
@@ -1600,9 +1626,9 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_zpoisson(const GangBox *__
     const GangBox &B = g[blockIdx.z];
     const uint3 b = make_uint3(blockIdx.x, blockIdx.y, 0);
     if constexpr (REGS)
-        k_zpoisson_r_body(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
+        k_zpoisson_r_body<false>(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
     else
-        k_zpoisson_body(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
+        k_zpoisson_body<false>(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
 }
 
 template <bool REGS>
@@ -1899,6 +1925,87 @@ __global__ void __launch_bounds__(BLK) k_ccl_volume(const unsigned *__restrict__
             else hi = mid;
         }
         atomicAdd(&vol[lo], 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Random-phase initial velocity (the turbulent start the reference reads from
+// input/{u,v,w}.dat, main.f90:234-239): a random vector potential A drawn on
+// the half spectrum, brought to the grid by the inverse passes of the Poisson
+// solver, and u = curl A with the staggered grid's forward differences, so
+// the discrete divergence of main.f90:723-725 vanishes up to rounding.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
+{
+    // SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): a counter-based hash, so
+    // each spectral coefficient depends only on (seed, component, position).
+    unsigned long long z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double unit53(unsigned long long h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+// A_comp(q, ky, kz) = amp(k) g e^{i theta}: amp = sqrt(exp(-2 (k/kp)^2) / (4 pi))
+// for 0 < k <= kmax, g standard normal (Box-Muller), theta uniform.
+__global__ void __launch_bounds__(BLK) k_ic_spectrum(double2 *__restrict__ spec, int n, int ld, int comp,
+                                                     unsigned long long seedmix, double kp, double kmax)
+{
+    const int nh = n / 2 + 1;
+    const size_t total = (size_t)nh * n * n;
+    const double twopi = 6.283185307179586;
+    for (size_t e = (size_t)blockIdx.x * BLK + threadIdx.x; e < total; e += (size_t)gridDim.x * BLK) {
+        const int q = (int)(e % nh);
+        const int j = (int)((e / nh) % n);
+        const int k = (int)(e / ((size_t)nh * n));
+        const double kq = q, ky = j < n / 2 ? j : j - n, kz = k < n / 2 ? k : k - n;
+        const double kk = __builtin_sqrt((kq * kq + ky * ky) + kz * kz);
+        const unsigned long long idx = (((unsigned long long)comp * n + k) * n + j) * nh + q;
+        const unsigned long long key = seedmix ^ (idx * 4ull);
+        const double u1 = unit53(splitmix64(key)), u2 = unit53(splitmix64(key + 1ull)),
+                     u3 = unit53(splitmix64(key + 2ull));
+        const double g = sqrt(-2.0 * log(1.0 - u1)) * cos(twopi * u2);
+        const double th = twopi * u3;
+        const double amp = (kk > 0.0 && kk <= kmax) ? sqrt(exp(-2.0 * ((kk / kp) * (kk / kp))) / (4.0 * 3.141592653589793)) : 0.0;
+        const double a = amp * g;
+        spec[spec_row(k, j, n, ld) + q] = make_double2(a * cos(th), a * sin(th));
+    }
+}
+
+// u = d_y A_z - d_z A_y, v = d_z A_x - d_x A_z, w = d_x A_y - d_y A_x (forward
+// differences / dx).  SUM: per-x-line sums of u^2 + v^2 + w^2 into lsum;
+// otherwise write s*u, s*v, s*w.
+template <typename T, bool SUM>
+__global__ void __launch_bounds__(BLK) k_ic_curl(const double *__restrict__ ax, const double *__restrict__ ay,
+                                                 const double *__restrict__ az, T *__restrict__ u, T *__restrict__ v,
+                                                 T *__restrict__ w, double *__restrict__ lsum, int n, double dx,
+                                                 double s)
+{
+    const int lane = threadIdx.x & 63;
+    const size_t line = ((size_t)blockIdx.x * BLK + threadIdx.x) >> 6;
+    if (line >= (size_t)n * n) return;
+    const int j = (int)(line % n), k = (int)(line / n);
+    const int jp = j + 1 == n ? 0 : j + 1, kp = k + 1 == n ? 0 : k + 1;
+    double acc = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const int ip = i + 1 == n ? 0 : i + 1;
+        const size_t id = IDX(i, j, k);
+        const double a0x = ax[id], a0y = ay[id], a0z = az[id];
+        const double uu = (az[IDX(i, jp, k)] - a0z) / dx - (ay[IDX(i, j, kp)] - a0y) / dx;
+        const double vv = (ax[IDX(i, j, kp)] - a0x) / dx - (az[IDX(ip, j, k)] - a0z) / dx;
+        const double ww = (ay[IDX(ip, j, k)] - a0y) / dx - (ax[IDX(i, jp, k)] - a0x) / dx;
+        if constexpr (SUM) {
+            acc = acc + ((uu * uu + vv * vv) + ww * ww);
+        } else {
+            u[id] = (T)(uu * s);
+            v[id] = (T)(vv * s);
+            w[id] = (T)(ww * s);
+        }
+    }
+    if constexpr (SUM) {
+        acc = wave_sum(acc);
+        if (lane == 0) lsum[line] = acc;
     }
 }
 
@@ -2339,6 +2446,48 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
         hipLaunchKernelGGL(kg_project<double>, gp, dim3(BLK), 0, s, g);
     hipLaunchKernelGGL(kg_reduce_planes, dim3(n, 3, nbox), dim3(64), 0, s, g);
     hipLaunchKernelGGL(kg_finalize, dim3(nbox), dim3(64), 0, s, g);
+}
+
+// Inverse 3-D transform of the half spectrum into `out` (n^3, x fastest):
+// z inverse, y inverse, x complex-to-real; unnormalised.
+void launch_ic_potential(const Fields &f, const Consts &c, const FftPlan &pl, const Tuning &tu, int comp,
+                         unsigned long long seedmix, double kp, double kmax, double *out, hipStream_t s)
+{
+    const size_t total = (size_t)(c.n / 2 + 1) * c.n * c.n;
+    const unsigned g = nblocks(total, BLK) < 65536 ? nblocks(total, BLK) : 65536;
+    hipLaunchKernelGGL(k_ic_spectrum, dim3(g), dim3(BLK), 0, s, f.spec, c.n, f.ld, comp, seedmix, kp, kmax);
+    const int nh = c.n / 2 + 1;
+    if (plan_regs(pl)) {
+        const int tkx = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl);
+        const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+        hipLaunchKernelGGL(k_zinv_r, dim3((nh + tkx - 1) / tkx, c.n), dim3(tkx * (c.n / 8)), shm, s, f.spec, f.tw,
+                           f.ktab, f.sc, pl, f.ld, tkx);
+    } else {
+        const int tkx = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
+        const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
+        hipLaunchKernelGGL(k_zinv, dim3((nh + tkx - 1) / tkx, c.n), dim3(fft_threads(pl, tkx)), shm, s, f.spec,
+                           f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+    }
+    LaunchInfo li;
+    launch_ypass(f, c, pl, tu, 1, s, &li);
+    Fields g2 = f;
+    g2.p = out;
+    launch_xinv(g2, c, pl, tu, s, &li);
+}
+
+void launch_ic_curl(const Consts &c, const double *ax, const double *ay, const double *az, void *u, void *v,
+                    void *w, double *lsum, double dx, double scale, bool sum, hipStream_t s)
+{
+    const dim3 grid(nblocks((size_t)c.n * c.n * 64, BLK));
+    if (sum)
+        hipLaunchKernelGGL((k_ic_curl<double, true>), grid, dim3(BLK), 0, s, ax, ay, az, (double *)nullptr,
+                           (double *)nullptr, (double *)nullptr, lsum, c.n, dx, scale);
+    else if (c.mixed)
+        hipLaunchKernelGGL((k_ic_curl<float, false>), grid, dim3(BLK), 0, s, ax, ay, az, (float *)u, (float *)v,
+                           (float *)w, lsum, c.n, dx, scale);
+    else
+        hipLaunchKernelGGL((k_ic_curl<double, false>), grid, dim3(BLK), 0, s, ax, ay, az, (double *)u, (double *)v,
+                           (double *)w, lsum, c.n, dx, scale);
 }
 
 }  // namespace mhit

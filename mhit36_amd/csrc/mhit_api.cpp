@@ -570,6 +570,62 @@ extern "C" int mhit_init_fields(mhit_ctx *c)
     return MHIT_OK;
 }
 
+// Random-phase turbulent start on the device: u = curl A of a random vector
+// potential with E(k) ~ k^4 exp(-2 (k/k_peak)^2), discretely divergence-free,
+// scaled to urms.  The reference reads such a field from input/{u,v,w}.dat
+// (inflow = 1, main.f90:234-239, readwrite.f90:107-194); this makes it in
+// place, at any size the device holds, in a few passes.
+extern "C" int mhit_init_random_velocity(mhit_ctx *c, unsigned long long seed, double k_peak, double urms)
+{
+    if (!c || !(k_peak > 0.0) || !(urms > 0.0)) return MHIT_ERR_ARG;
+    if (c->halted) return c->halt_status;
+    const int n = c->kc.n;
+    const size_t n3 = (size_t)n * n * n;
+    double kmax = (double)std::min(n / 2 - 1, (int)(6.0 * k_peak));
+    if (kmax < 1.0) return MHIT_ERR_BAD_N;
+    double *A = nullptr;
+    if (hipMalloc((void **)&A, 3 * n3 * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return MHIT_ERR_OOM;
+    }
+    // the key of every draw is splitmix64(seed) ^ 4 * (position index)
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    const unsigned long long seedmix = z ^ (z >> 31);
+    hipStream_t st = c->stream;
+    for (int comp = 0; comp < 3; comp++)
+        launch_ic_potential(c->f, c->kc, c->plan, c->tune, comp, seedmix, k_peak, kmax, A + comp * n3, st);
+    const double dx = c->cs.dx;
+    launch_ic_curl(c->kc, A, A + n3, A + 2 * n3, nullptr, nullptr, nullptr, c->f.lsum, dx, 1.0, true, st);
+    std::vector<double> ls((size_t)n * n);
+    int rc = MHIT_OK;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(ls.data(), c->f.lsum, ls.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        rc = MHIT_ERR_HIP;
+    double scale = 0.0;
+    if (rc == MHIT_OK) {
+        long double sum = 0.0L;
+        for (double x : ls) sum += x;
+        const double rms = std::sqrt((double)sum / (3.0 * (double)n3));
+        if (!(rms > 0.0)) rc = MHIT_ERR_ARG;
+        scale = urms / rms;
+    }
+    if (rc == MHIT_OK) {
+        launch_ic_curl(c->kc, A, A + n3, A + 2 * n3, c->f.u, c->f.v, c->f.w, c->f.lsum, dx, scale, false, st);
+        // the stored fields are now the fields themselves: pending means zero
+        const double zero3[3] = {0.0, 0.0, 0.0};
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&c->f.sc->umean, zero3, sizeof(zero3), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = MHIT_ERR_HIP;
+    }
+    (void)hipFree(A);
+    if (rc == MHIT_OK) c->loaded = true;
+    return rc;
+}
+
 // A restart (main.f90:318-323): the step counter, an Euler next step, and
 // gumax = 1.0, the value the reference starts every run with (it sets the
 // lagged gamma of the first step, main.f90:321, 362).  The pending means of

@@ -13,17 +13,19 @@ from .solver import Solver, default_params
 SIZES = (64, 96, 128, 192, 256)
 
 
-def c5_stream(count=32, seed=2024, rate_hz=40.0, steps=(200, 2000), sizes=SIZES, precision=1):
+def c5_stream(count=32, seed=2024, rate_hz=40.0, steps=(200, 2000), sizes=SIZES, precision=1, cover=False):
     """Requests: arrival time (s), parameters, step count and initial phi.
     N from `sizes`, nu in [0.002, 0.02], half of the boxes with ACDI drops
     (8-64 drops, radius 0.15-0.3); velocity is the built-in Taylor-Green field,
-    dt = 0.05 dx (CFL 0.05 at umax 1)."""
+    dt = 0.05 dx (CFL 0.05 at umax 1).  cover=True gives the first
+    len(sizes) requests every size once, in a seeded order."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
+    first = [int(x) for x in rng.permutation(np.asarray(sizes))] if cover else []
     for q in range(count):
         t += rng.exponential(1.0 / rate_hz)
-        n = int(rng.choice(sizes))
+        n = first[q] if q < len(first) else int(rng.choice(sizes))
         nu = float(rng.uniform(0.002, 0.02))
         acdi = bool(rng.random() < 0.5)
         nsteps = int(np.exp(rng.uniform(np.log(steps[0]), np.log(steps[1]))))
@@ -76,8 +78,10 @@ def run_one_at_a_time(reqs, autotune=True):
     return time.perf_counter() - t0, out
 
 
-def run_ensemble(reqs, follow_arrivals=True, **cfg):
-    """Submit at the arrival times (host clock) and drive the ensemble."""
+def run_ensemble(reqs, follow_arrivals=True, fields=(1,), **cfg):
+    """Submit at the arrival times (host clock) and drive the ensemble.
+    Returns the wall time, each request's final fields (`fields`; only u,
+    as an array, when fields == (1,)), the step latencies and the results."""
     ens = Ensemble(**cfg)
     t0 = time.perf_counter()
     ids = {}
@@ -94,7 +98,11 @@ def run_ensemble(reqs, follow_arrivals=True, **cfg):
         time.sleep(2e-5)            # leave the core to the HIP runtime's threads
     wall = time.perf_counter() - t0
     res = {r["id"]: r for r in ens.results()}
-    out = {q: ens.get_field(i, 1) for q, i in ids.items()}
+    if tuple(fields) == (1,):
+        out = {q: ens.get_field(i, 1) for q, i in ids.items()}
+    else:
+        out = {q: {f: ens.get_field(i, f) for f in fields
+                   if f != 5 or reqs[q]["params"].get("phiflag", 1)} for q, i in ids.items()}
     lat = ens.latency()
     ens.close()
     return wall, out, lat, [res[ids[q]] for q in range(len(reqs))]

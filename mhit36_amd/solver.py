@@ -125,6 +125,13 @@ class Solver:
             raise MhitError(rc, "mhit_step")
         return d
 
+    def init_random_velocity(self, seed=1234, k_peak=4.0, urms=1.0):
+        """Random-phase, discretely divergence-free turbulent start made on the
+        device (mhit_init_random_velocity): E(k) ~ k^4 exp(-2 (k/k_peak)^2),
+        scaled to urms."""
+        check(self._lib.mhit_init_random_velocity(self._h, int(seed), float(k_peak), float(urms)),
+              "init_random_velocity")
+
     def reset_time(self, tstart):
         check(self._lib.mhit_reset_time(self._h, int(tstart)), "reset_time")
 

@@ -760,7 +760,6 @@ static void phase_field(orc_state *s, double gamma)
      * the sharpening flux a*n (a = 0.25*(1 - tanh^2(psi/(2 eps)))) is formed
      * per cell once and used by both neighbours (same value either way). */
     const double tiny = (double)1.0E-16f;
-    const double sharp_c = 1.0 / (1.0 + 2.0 * en);
     #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
         int kp = (k + 1) % n, km = (k + n - 1) % n;
@@ -772,19 +771,21 @@ static void phase_field(orc_state *s, double gamma)
                 double gy = psi[IDX(i, jp, k, n)] - psi[IDX(i, jm, k, n)];
                 double gz = psi[IDX(i, j, kp, n)] - psi[IDX(i, j, km, n)];
                 double normod = 1.0 / (sqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                double th;
+                double a;
                 if (s->sharpening) {
-                    /* tanh(psi/(2 eps)) = tanh(log(q)/2) = (q-1)/(q+1) = (2 val - 1)/(1 + 2 enum)
-                     * for psi = eps log q, q = (val + enum)/(1 - val + enum); a NaN psi
-                     * (phi below -enum, Q7) stays NaN as tanh(NaN) does. */
+                    /* psi = eps log(A/B), A = val + enum, B = (1 - val) + enum, so
+                     * 1/4 (1 - tanh^2(psi/(2 eps))) = A B / (A + B)^2 exactly; like the
+                     * tanh form it goes to 0 with A (phi -> -enum).  A NaN psi (phi below
+                     * -enum, Q7) stays NaN, as tanh(NaN) does. */
                     double f = phi[IDX(i, j, k, n)];
                     double val = f < 1.0 ? f : 1.0;
+                    double A = val + en, B = (1.0 - val) + en, S = A + B;
                     double ps = psi[IDX(i, j, k, n)];
-                    th = (ps != ps) ? ps : (2.0 * val - 1.0) * sharp_c;
+                    a = (ps != ps) ? ps : (A * B) / (S * S);
                 } else {
-                    th = orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi);
+                    double th = orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi);
+                    a = 0.25 * (1.0 - th * th);
                 }
-                double a = 0.25 * (1.0 - th * th);
                 size_t id = IDX(i, j, k, n);
                 s->sx[id] = a * (gx * normod);
                 s->sy[id] = a * (gy * normod);

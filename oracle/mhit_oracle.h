@@ -83,10 +83,10 @@ int    orc_step(orc_state *s, int nsteps, orc_diag *d);
 void   orc_get_consts(const orc_state *s, orc_consts *c);
 /* How the sharpening factor 1/4 (1 - tanh^2(psi/(2 eps))) of main.f90:463-465
  * is evaluated.  0 (the default): tanh of psi as the Fortran writes it.
- * 1: through the identity tanh(psi/(2 eps)) = (2 val - 1)/(1 + 2 enum), exact
- * for psi = eps log((val + enum)/(1 - val + enum)) (main.f90:388-389), as
- * (2 val - 1) * (1/(1 + 2 enum)), NaN where psi is NaN: the form the HIP
- * path evaluates.  The two differ by rounding only. */
+ * 1: through the identity 1/4 (1 - tanh^2(psi/(2 eps))) = A B / (A + B)^2,
+ * A = val + enum, B = (1 - val) + enum, exact for psi = eps log(A/B)
+ * (main.f90:388-389); NaN where psi is NaN: the form the HIP path evaluates.
+ * The two differ by rounding only. */
 void   orc_set_sharpening(orc_state *s, int form);
 void   orc_get_tables(const orc_state *s, double *x, double *ktab);
 int    orc_debug_buffer(const orc_state *s, int which, double *dst);

@@ -121,8 +121,9 @@ class Oracle:
 
     sharpening: how the ACDI sharpening factor 1/4 (1 - tanh^2(psi/(2 eps)))
     is evaluated -- "identity" (default: the form the HIP path evaluates,
-    (2 val - 1)/(1 + 2 enum) for tanh(psi/(2 eps))) or "literal" (tanh of
-    psi as main.f90:463-465 writes it).  They differ by rounding only."""
+    A B / (A + B)^2 with A = val + enum, B = 1 - val + enum) or "literal"
+    (tanh of psi as main.f90:463-465 writes it).  They differ by rounding
+    only."""
 
     def __init__(self, lib_path=None, sharpening="identity", **kw):
         self.lib = load(lib_path)

@@ -1,0 +1,58 @@
+"""The serving stream of the configuration list on the device: Poisson-arrival
+requests of independent periodic boxes, N in {64, 96, 128, 192, 256} (each
+size at least once), nu in [0.002, 0.02], half with ACDI drops (8-64 drops,
+radius 0.15-0.3), 200-2000 steps each, mixed precision, admitted by the HBM
+budget while earlier boxes run (continuous batching, gangs for the small
+sizes).  One request is switched to FP64.
+
+Every box's fields in the mix equal the same box run alone, bit for bit, and
+the FP64 box equals the oracle bit for bit."""
+import numpy as np
+import pytest
+
+from mhit36_amd import Solver, serve
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream():
+    reqs = serve.c5_stream(count=12, seed=7, rate_hz=40.0, steps=(200, 2000), cover=True)
+    assert {r["params"]["nx"] for r in reqs} == set(serve.SIZES)
+    assert any(r["phi"] is not None for r in reqs) and any(r["phi"] is None for r in reqs)
+    # the FP64 box: the first 64^3 box with drops, cut to 200 steps for the oracle
+    q64 = min((q for q, r in enumerate(reqs) if r["params"]["nx"] == 64),
+              key=lambda q: (reqs[q]["phi"] is None, q))
+    reqs[q64]["params"] = {**reqs[q64]["params"], "precision": 0}
+    reqs[q64]["nsteps"] = 200
+    return reqs, q64
+
+
+def _alone(r):
+    with Solver(**r["params"]) as s:
+        s.init_fields()
+        if r["phi"] is not None:
+            s.set_field(5, r["phi"])
+        s.step(r["nsteps"], raise_on_stop=False)
+        return {f: s.get_field(f) for f in ((1, 2, 3, 4, 5) if r["params"]["phiflag"] else (1, 2, 3, 4))}
+
+
+def test_c5_stream_same_alone_and_in_mix_and_fp64_box_equals_oracle():
+    reqs, q64 = _stream()
+    print("C5 stream:", [(r["params"]["nx"], r["nsteps"], r["params"]["phiflag"], r["params"]["precision"])
+                         for r in reqs], flush=True)
+    wall, out, lat, res = serve.run_ensemble(reqs, follow_arrivals=True, fields=(1, 2, 3, 4, 5), inflight=2)
+    for q, r in enumerate(reqs):
+        assert res[q]["status"] == 0 and res[q]["steps_done"] == r["nsteps"], (q, res[q])
+        ref = _alone(r)
+        for f, a in ref.items():
+            assert np.array_equal(out[q][f], a), (q, r["params"]["nx"], f)
+    r = reqs[q64]
+    o = orc.Oracle(**r["params"])
+    o.init_ic()
+    if r["phi"] is not None:
+        o.set_field(5, r["phi"])
+    o.step(r["nsteps"])
+    for f in ((1, 2, 3, 4, 5) if r["params"]["phiflag"] else (1, 2, 3, 4)):
+        assert np.array_equal(out[q64][f], o.get_field(f)), ("oracle", f)
+    assert lat["p99_ms"] >= lat["p50_ms"] > 0

@@ -147,9 +147,9 @@ def test_euler_then_ab2():
 
 @pytest.mark.parametrize("fp32_rhs", [0, 1])
 def test_sharpening_identity_matches_literal_tanh(fp32_rhs):
-    """The HIP path evaluates tanh(psi/(2 eps)) through the identity
-    (2 val - 1)/(1 + 2 enum); the oracle has both forms, which differ by
-    rounding only.  Over 200 steps of a forced flow with drops:
+    """The HIP path evaluates the sharpening factor 1/4 (1 - tanh^2(psi/(2 eps)))
+    through the identity A B / (A + B)^2 (A = val + enum, B = 1 - val + enum);
+    the oracle has both forms, which differ by rounding only.  Over 200 steps of a forced flow with drops:
     - with an FP64 Poisson right-hand side the fields stay within 1e-13;
     - with the reference's REAL(4) right-hand side (Q3, main.f90:738) one
       rounding difference can flip a float rounding of rhsp, which moves p by

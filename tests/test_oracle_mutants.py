@@ -24,11 +24,14 @@ SRC = os.path.join(ROOT, "oracle", "mhit_oracle.c")
 MUTANTS = [
     ("sharpening_sign", "double r = rphi[id] - gamma * ((X + Y) + Z);",
      "double r = rphi[id] + gamma * ((X + Y) + Z);", "sphere_equilibrium"),
-    ("tanh_factor", "double a = 0.25 * (1.0 - th * th);",
-     "double a = 0.5 * (1.0 - th * th);", "sphere_equilibrium"),
+    ("tanh_factor", "a = 0.25 * (1.0 - th * th);",
+     "a = 0.5 * (1.0 - th * th);", "sphere_equilibrium", "literal"),
+    ("identity_factor", "a = (ps != ps) ? ps : (A * B) / (S * S);",
+     "a = (ps != ps) ? ps : (2.0 * A * B) / (S * S);", "sphere_equilibrium"),
     ("tanh_argument", "orc_tanh((0.5 * psi[IDX(i, j, k, n)]) * epsi)",
      "orc_tanh(psi[IDX(i, j, k, n)] * epsi)", "sphere_equilibrium", "literal"),
-    ("identity_offset", "(2.0 * val - 1.0) * sharp_c", "(2.0 * val - 0.5) * sharp_c", "sphere_equilibrium"),
+    ("identity_sum", "double A = val + en, B = (1.0 - val) + en, S = A + B;",
+     "double A = val + en, B = (1.0 - val) + en, S = A + 2.0 * B;", "sphere_equilibrium"),
     ("diffusion_y_no_eps", "double dyy = eps * ((fyp - 2.0 * f0) + fym) * ddxi;",
      "double dyy = ((fyp - 2.0 * f0) + fym) * ddxi;", "sphere_equilibrium"),
     ("normal_y_one_sided", "double gy = psi[IDX(i, jp, k, n)] - psi[IDX(i, jm, k, n)];",
