@@ -74,42 +74,70 @@ def params_for(n, phiflag, dt):
                 gamma_factor=1.2)
 
 
-def cpu_baseline(n, phiflag, fields, dt, steps=2):
-    """Oracle (CPU restatement, C + OpenMP) on the same workload: `steps`
-    steps, the last one timed."""
+BOX_CPU_SHARE = 16      # host threads one GPU's process may use on the GPU box
+
+
+def host_cores():
+    """(sockets, physical cores) of this host from lscpu's parseable listing
+    (unique (socket, core) pairs), or (None, os.cpu_count()) without lscpu."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=SOCKET,CORE"], capture_output=True, text=True, check=True).stdout
+        pairs = {tuple(line.split(",")) for line in out.splitlines() if line and not line.startswith("#")}
+        return len({p[0] for p in pairs}), len(pairs)
+    except (OSError, subprocess.CalledProcessError):
+        return None, os.cpu_count() or 1
+
+
+def _oracle_step_times(n, phiflag, fields, dt, threads, untimed, timed):
+    """Wall time (ms) of each of `timed` oracle steps after `untimed` ones
+    (the first step is the Euler start; the timed ones are AB2 steps)."""
     from oracle import orc
-    threads = min(16, os.cpu_count() or 1)
     orc.set_threads(threads)
     o = orc.Oracle(**params_for(n, phiflag, dt))
     for f, a in fields.items():
         o.set_field(f, a)
-    o.step(steps - 1)
-    t0 = time.perf_counter()
-    o.step(1)
-    ms = (time.perf_counter() - t0) * 1e3
+    o.step(untimed)
+    ts = []
+    for _ in range(timed):
+        t0 = time.perf_counter()
+        o.step(1)
+        ts.append((time.perf_counter() - t0) * 1e3)
     o.close()
-    return {"value": round(ms, 3), "unit": "ms/timestep", "cores": threads, "kind": "port",
+    return ts
+
+
+def cpu_baseline(n, phiflag, fields, dt, untimed=2, timed=5):
+    """Oracle (the CPU restatement, C + OpenMP, gcc -O3) on the same workload:
+    median wall time of `timed` steps after `untimed` ones, on the host's
+    physical cores (lscpu) up to one GPU's share of the box."""
+    import statistics
+    sockets, phys = host_cores()
+    threads = max(1, min(phys, BOX_CPU_SHARE))
+    ts = _oracle_step_times(n, phiflag, fields, dt, threads, untimed, timed)
+    return {"value": round(statistics.median(ts), 3), "unit": "ms/timestep", "cores": threads, "kind": "port",
+            "host": {"sockets": sockets, "physical_cores": phys, "threads_used": threads},
             "sample": f"oracle/liboracle.so, {n}^3 {'NS+ACDI' if phiflag else 'NS'} workload above, "
-                      f"{steps} steps, the last one timed, {threads} OpenMP threads"}
+                      f"median of {timed} steps after {untimed} untimed, {threads} OpenMP threads"}
 
 
-def cpu_baseline_1thread(n=256, phiflag=1):
-    """The oracle on one host thread, on the same generators at 256^3 (one
-    512^3 step would take ~40 s on one thread): one untimed step, one timed."""
-    from oracle import orc
-    u, v, w, phi, dt = make_inputs(n, phiflag, ndrops=64, radius=0.2)
-    orc.set_threads(1)
-    o = orc.Oracle(**params_for(n, phiflag, dt))
-    for f, a in {1: u, 2: v, 3: w, 5: phi}.items():
-        o.set_field(f, a)
-    o.step(1)
-    t0 = time.perf_counter()
-    o.step(1)
-    ms = (time.perf_counter() - t0) * 1e3
-    o.close()
-    return {"value": round(ms, 3), "unit": "ms/timestep", "cores": 1, "kind": "port",
-            "sample": f"oracle/liboracle.so, {n}^3 NS+ACDI (same generators, 64 drops R=0.2), "
-                      "2 steps, the last one timed, 1 thread"}
+def cpu_baseline_1thread(n_big=512):
+    """The oracle on one host thread.  One 512^3 step takes tens of seconds on
+    one thread, so the figure for the bench box is extrapolated from measured
+    128^3 and 256^3 steps (same generators, 8-64 drops): t(512) = t(256) *
+    (t(256) / t(128)), the growth per doubling measured one size down."""
+    import statistics
+    meas = {}
+    for n, nd, r in ((128, 8, 0.3), (256, 64, 0.2)):
+        u, v, w, phi, dt = make_inputs(n, 1, ndrops=nd, radius=r)
+        ts = _oracle_step_times(n, 1, {1: u, 2: v, 3: w, 5: phi}, dt, 1, 1, 3)
+        meas[n] = statistics.median(ts)
+    est = meas[256] * (meas[256] / meas[128])
+    return {"value": round(est, 1), "unit": "ms/timestep", "cores": 1, "kind": "port",
+            "measured_ms": {f"{k}^3": round(v, 3) for k, v in meas.items()},
+            "sample": f"oracle/liboracle.so on 1 thread, NS+ACDI: median of 3 steps after 1 untimed at 128^3 "
+                      f"(8 drops R=0.3) and 256^3 (64 drops R=0.2); {n_big}^3 extrapolated as "
+                      "t(256) * t(256) / t(128)"}
 
 
 def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):

@@ -68,7 +68,9 @@ def random_velocity(n, seed=1234, k_peak=4.0, urms=1.0, kmax=None):
 
 
 def drop_centres(ndrops, radius, eps, seed=4321, lx=None, max_tries=200000):
-    """Random centres with periodic separation >= 2R + 8 eps."""
+    """Random centres in [0, lx) with periodic separation >= 2R + 8 eps over
+    the period lx (default 2 pi; separation over 2 pi implies it over the
+    slightly longer period n dx that drops() uses)."""
     rng = np.random.default_rng(seed)
     lx = lx if lx is not None else 8.0 * np.arctan(1.0)
     sep = 2 * radius + 8 * eps
@@ -92,9 +94,14 @@ def drop_centres(ndrops, radius, eps, seed=4321, lx=None, max_tries=200000):
 
 
 def drops(n, centres, radius, eps):
-    """Phase field of spheres at `centres` on the node grid of main.f90:149-153."""
+    """Phase field of spheres at `centres` on the node grid of main.f90:149-153.
+
+    Distances are periodic over the n cells of the box, whose period is
+    n dx = 2 pi n / (n - 1) on this grid (node n-1 sits at 2 pi, one cell short
+    of node 0's periodic image), so a drop that crosses the seam is as whole
+    as one in the middle of the box."""
     x, dx = grid(n)
-    lx = 8.0 * np.arctan(1.0)
+    lx = n * dx
     phi = np.zeros((n, n, n), order="F")
     half = radius + 40 * eps      # profile below 1e-17 beyond: no jump at the box edge
     w = min(int(np.ceil(half / dx)) + 1, n // 2)     # beyond n/2 the box only repeats cells
