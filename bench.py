@@ -158,7 +158,17 @@ def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
     return round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)
 
 
-def ensemble_leg(count=16, steps=(200, 800), inflight=2):
+def request_latency(res):
+    """Request-level latency: submit -> finish (ensemble clock) per request,
+    p50 and p99 over the requests (nearest rank)."""
+    import numpy as np
+    lat = np.sort(np.array([r["finish_ms"] - r["submit_ms"] for r in res], dtype=float))
+    rank = lambda q: float(lat[min(len(lat) - 1, max(0, int(np.ceil(q * len(lat))) - 1))])
+    return {"request_latency_p50_ms": round(rank(0.50), 3), "request_latency_p99_ms": round(rank(0.99), 3),
+            "requests": int(len(lat))}
+
+
+def ensemble_leg(count=16, steps=(200, 800), inflight=2, single_box_gcell_steps_s=None):
     """The serving stream: `count` seeded Poisson-arrival requests (40/s) of
     independent boxes, N in {64, 96, 128, 192, 256}, nu in [0.002, 0.02],
     half with ACDI drops, mixed precision.  Run one box at a time (create,
@@ -181,6 +191,10 @@ def ensemble_leg(count=16, steps=(200, 800), inflight=2):
             "one_at_a_time_gcell_steps_s": round(cells / t_one / 1e9, 3),
             "speedup": round(t_one / wall, 3), "step_latency_p50_ms": round(lat["p50_ms"], 4),
             "step_latency_p99_ms": round(lat["p99_ms"], 4), "steps_timed": lat["steps"],
+            **request_latency(res),
+            "single_box_ceiling_gcell_steps_s": single_box_gcell_steps_s,
+            "fraction_of_single_box_ceiling": (round(cells / wall / 1e9 / single_box_gcell_steps_s, 3)
+                                               if single_box_gcell_steps_s else None),
             "same_alone_and_in_mix": bool(same), "status": sorted(set(r["status"] for r in res)),
             "inflight": inflight, "small_boxes": small}
 
@@ -203,6 +217,7 @@ def small_box_leg(count=32, steps=(200, 800)):
         out[key] = {"ensemble_s": round(wall, 4), "speedup": round(t_one / wall, 3),
                     "throughput_gcell_steps_s": round(cells / wall / 1e9, 3),
                     "step_latency_p50_ms": round(lat["p50_ms"], 4), "step_latency_p99_ms": round(lat["p99_ms"], 4),
+                    **request_latency(res),
                     "same_alone_and_in_mix": bool(all(np.array_equal(out_one[q], o[q]) for q in out_one))}
     return out
 
@@ -303,7 +318,9 @@ def main():
 
     ens = None
     if not args.no_ensemble:
-        ens = ensemble_leg()
+        # the rate one large box reaches on this GPU: 512^3 cells per mixed step
+        ceiling = round(n ** 3 / (mixed["ms_per_step"] * 1e-3) / 1e9, 3) if mixed else None
+        ens = ensemble_leg(single_box_gcell_steps_s=ceiling)
         print(f"[bench] ensemble {ens['ensemble_s']:.3f}s vs one at a time {ens['one_at_a_time_s']:.3f}s, "
               f"p99 {ens['step_latency_p99_ms']:.3f} ms", file=sys.stderr, flush=True)
 

@@ -33,25 +33,38 @@ __device__ __forceinline__ uint3 xcd_block()
 // ---------------------------------------------------------------------------
 // Phase field, main.f90:338-486, fused into one pass (A4-A10).
 //
-// A block owns an ATX x ATY column of cells and marches kz planes in z.  LDS
-// rings carry what neighbouring cells share:
-//   psi (A6) on an (ATX+4) x (ATY+4) box, planes k..k+2;
-//   the sharpening flux s = a*n (A7-A9; a = 0.25(1 - tanh^2(psi/(2 eps))), n
-//   the normalised undivided normal) on an (ATX+2) x (ATY+2) box: s_x, s_y
-//   for planes k, k+1 and s_z for planes k-1..k+1;
-//   phi on the (ATX+2) x (ATY+2) box, planes k-1..k+2.
-// Iteration k adds psi(k+2) from phi values loaded one iteration earlier,
-// forms s(k+1), then the output plane k: convective and diffusive terms (A4,
-// A5), the sharpening divergence, and phi^{n+1} = phi + dt*rhs (A10) written
-// to a second buffer.  u, v, w of plane k+1 are loaded while plane k is
-// computed.  HBM traffic: u, v, w, phi read once (plus halos), phi^{n+1}
-// written once; psi and s never leave LDS.
+// A block owns an ATX x ATY column of cells and marches kz planes in z; thread
+// t owns the cell (t % ATX, t / ATX) of the tile in every plane.  What a cell
+// needs from its own column (psi at k-1..k+1 for the z part of the normal, phi
+// at k-1..k+1, the z sharpening flux at k-1 and k+1) stays in the owning
+// thread's registers; only what x/y neighbours share goes through LDS, one
+// plane of each, double-buffered by plane parity:
+//   psi (A6) on the tile plus a 2-cell ring (the 12 outer corners are never
+//   read), for the x/y parts of the normal;
+//   s_x, s_y of the sharpening flux s = a*n (A7-A9; a = 0.25(1 -
+//   tanh^2(psi/(2 eps))), n the normalised undivided normal) on the tile plus
+//   a 1-cell ring without corners, for the flux divergence;
+//   phi on the same cells, for the convective and diffusive stencils.
+// The ring cells are extra work items: the 96 cells of the 1-cell ring (psi,
+// the normal and s_x or s_y) go to threads HS0.., the 100 further psi-only
+// cells to threads HP0..; each such thread keeps its ring cell's column in
+// registers the way it keeps its own.
+// Iteration k: psi(k+2) from phi loaded one iteration earlier; s(k+1); the
+// output plane k: convective and diffusive terms (A4, A5), the sharpening
+// divergence and phi^{n+1} = phi + dt*rhs (A10) written to a second buffer;
+// then psi(k+2) and phi(k+1) into LDS, and one barrier.  u, v, w of plane k+1
+// and phi of plane k+3 are loaded while plane k is computed.  HBM traffic:
+// u, v, w, phi read once (plus halos), phi^{n+1} written once; psi and s
+// never leave the chip.
 // ---------------------------------------------------------------------------
 constexpr int ATX = 32, ATY = 16;
 constexpr int APX = ATX + 4, APY = ATY + 4;       // psi box
 constexpr int ASX = ATX + 2, ASY = ATY + 2;       // s / phi box
 constexpr int ATHREADS = ATX * ATY;
-static_assert(APX * APY <= 2 * ATHREADS && ASX * ASY <= 2 * ATHREADS, "two box elements per thread");
+constexpr int NHS = 2 * ATY + 2 * ATX;            // 1-cell ring without corners: psi, normal, s_x or s_y
+constexpr int NHP = 4 + 2 * ATY + 2 * ATX;        // its 4 corners and the 2-cell ring's sides: psi only
+constexpr int HS0 = ATHREADS / 2, HP0 = 3 * ATHREADS / 4;   // first thread of each ring's work
+static_assert(HS0 + NHS <= HP0 && HP0 + NHP <= ATHREADS, "one ring cell per thread at most");
 
 __device__ __forceinline__ int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
 
@@ -75,6 +88,25 @@ __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Tile coordinates (x in -2..ATX+1, y in -2..ATY+1) of ring work item h:
+// s ring (S) or psi-only ring (P).
+__device__ __forceinline__ int2 ring_s_cell(int h)
+{
+    if (h < ATY) return make_int2(-1, h);
+    if (h < 2 * ATY) return make_int2(ATX, h - ATY);
+    if (h < 2 * ATY + ATX) return make_int2(h - 2 * ATY, -1);
+    return make_int2(h - 2 * ATY - ATX, ATY);
+}
+__device__ __forceinline__ int2 ring_p_cell(int h)
+{
+    if (h < 4) return make_int2((h & 1) ? ATX : -1, (h & 2) ? ATY : -1);
+    h -= 4;
+    if (h < ATY) return make_int2(-2, h);
+    if (h < 2 * ATY) return make_int2(ATX + 1, h - ATY);
+    if (h < 2 * ATY + ATX) return make_int2(h - 2 * ATY, -2);
+    return make_int2(h - 2 * ATY - ATX, ATY + 1);
+}
+
 template <typename T>
 __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
@@ -82,12 +114,12 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     Consts c, int kz)
 {
     if (halted(sc)) return;
-    constexpr int NTAB = sizeof(T) == 8 ? MATH_TAB : 1;
+    constexpr int NTAB = sizeof(T) == 8 ? TAB_LHI + 2 * 129 : 1;   // the log part of the table
     __shared__ double tabL[NTAB];
-    __shared__ T psiL[3][APY * APX];
-    __shared__ T sxyL[2][2][ASY * ASX];
-    __shared__ T szL[3][ASY * ASX];
-    __shared__ T phL[4][ASY * ASX];
+    __shared__ T psiL[2][APY * APX];
+    __shared__ T sxL[2][ASY * ASX];
+    __shared__ T syL[2][ASY * ASX];
+    __shared__ T phL[2][ASY * ASX];
     const int n = c.n;
     const int tid = threadIdx.x;
     const int i0 = blk.x * ATX, j0 = blk.y * ATY, k0 = blk.z * kz;
@@ -97,143 +129,138 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const T tiny = (T)1.0E-16f;
     const size_t nn = (size_t)n * n;
 
-    // psi-box elements of this thread: x/y offset in the field and, when the
-    // element also lies in the inner (ATX+2) x (ATY+2) box, its index there.
-    size_t poff[2];
-    int pin[2];
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-        int e = tid + q * ATHREADS;
-        int bx = e % APX, by = e / APX;
-        poff[q] = (size_t)wrap(i0 - 2 + bx, n) + (size_t)n * wrap(j0 - 2 + by, n);
-        bool inner = e < APX * APY && bx >= 1 && bx <= ASX && by >= 1 && by <= ASY;
-        pin[q] = inner ? (by - 1) * ASX + (bx - 1) : -1;
-    }
-    T pre[2];
-    auto fetch_phi = [&](int m) {
-        const size_t zoff = nn * (size_t)wrap(m, n);
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-            if (tid + q * ATHREADS < APX * APY) pre[q] = phi[poff[q] + zoff];
-    };
-    auto psi_plane = [&](int m) {            // A6 on plane m from pre[]
-        const int slot = (m - k0 + 3) % 3;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            int e = tid + q * ATHREADS;
-            if (e < APX * APY) {
-                T f = pre[q];
-                T val = f < (T)1.0 ? f : (T)1.0;
-                if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
-                psiL[slot][e] = eps * tlog(tdiv(val + en, ((T)1.0 - val) + en), tabL);
-            }
-        }
-    };
-    auto phi_store = [&](int m) {            // phi of plane m into the inner ring
-        const int slot = (m - k0 + 4) & 3;
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-            if (pin[q] >= 0) phL[slot][pin[q]] = pre[q];
-    };
-    // The factor 1/4 (1 - tanh^2(psi/(2 eps))) of A9 from the cell's phi, no
-    // tanh: psi = eps log(A/B), A = val + enum, B = (1 - val) + enum, so it is
-    // A B / (A + B)^2 exactly (and, like the tanh form, goes to 0 with A as phi
-    // nears -enum).  A NaN psi (phi below -enum, Q7) stays NaN, as tanh(NaN)
-    // would.
-    auto s_plane = [&](int m) {              // A7 + A8 + a(psi) on plane m
-        const int ps = (m - k0 + 3) % 3, psm = (m - 1 - k0 + 3) % 3, psp = (m + 1 - k0 + 3) % 3;
-        const int sxy = (m - k0 + 2) & 1, sz = (m - k0 + 3) % 3, phs = (m - k0 + 4) & 3;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            int e = tid + q * ATHREADS;
-            if (e < ASX * ASY) {
-                int sx = e % ASX, sy = e / ASX;
-                int pc = (sy + 1) * APX + (sx + 1);
-                T gx = psiL[ps][pc + 1] - psiL[ps][pc - 1];
-                T gy = psiL[ps][pc + APX] - psiL[ps][pc - APX];
-                T gz = psiL[psp][pc] - psiL[psm][pc];
-                T normod = tdiv((T)1.0, tsqrt((gx * gx + gy * gy) + gz * gz) + tiny);
-                const T f = phL[phs][e], pc0 = psiL[ps][pc];
-                T val = f < (T)1.0 ? f : (T)1.0;
-                if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // as psi's FP32 clamp
-                const T A = val + en, B = ((T)1.0 - val) + en, S = A + B;
-                const T a = pc0 != pc0 ? pc0 : tdiv(A * B, S * S);
-                sxyL[sxy][0][e] = a * (gx * normod);
-                sxyL[sxy][1][e] = a * (gy * normod);
-                szL[sz][e] = a * (gz * normod);
-            }
-        }
-    };
-
+    // own cell
     const int tx = tid % ATX, ty = tid / ATX;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = (i < n) && (j < n);
-    const int ip = i + 1 >= n ? i + 1 - n : i + 1;
-    const int jp = j + 1 >= n ? j + 1 - n : j + 1;
-    const size_t cxy = (size_t)(i < n ? i : 0) + (size_t)n * (j < n ? j : 0);
-    const size_t xpo = (size_t)ip + (size_t)n * (j < n ? j : 0);
-    const size_t ypo = (size_t)(i < n ? i : 0) + (size_t)n * jp;
-    T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the next output plane
-    auto fetch_uvw = [&](int m) {
-        if (active) {
-            const size_t z = nn * (size_t)m;
-            nu0 = u[cxy + z]; nuxp = u[xpo + z];
-            nv0 = v[cxy + z]; nvyp = v[ypo + z];
-            nw0 = w[cxy + z];
-        }
+    const size_t own = (size_t)wrap(i, n) + (size_t)n * wrap(j, n);   // ghost cells (i or j >= n) wrap
+    const int pI = (ty + 2) * APX + (tx + 2), sI = (ty + 1) * ASX + (tx + 1);
+    // ring cell: kind 1 = s ring with s_x, 2 = s ring with s_y, 3 = psi only
+    int hk = 0;
+    int2 hc = make_int2(0, 0);
+    if (tid >= HS0 && tid < HS0 + NHS) { hk = tid - HS0 < 2 * ATY ? 1 : 2; hc = ring_s_cell(tid - HS0); }
+    else if (tid >= HP0 && tid < HP0 + NHP) { hk = 3; hc = ring_p_cell(tid - HP0); }
+    const size_t hoff = (size_t)wrap(i0 + hc.x, n) + (size_t)n * wrap(j0 + hc.y, n);
+    const int hP = (hc.y + 2) * APX + (hc.x + 2), hS = (hc.y + 1) * ASX + (hc.x + 1);
+
+    auto ld_phi = [&](size_t off, int m) { return phi[off + nn * (size_t)wrap(m, n)]; };
+    auto psi_of = [&](T f) {                  // A6
+        T val = f < (T)1.0 ? f : (T)1.0;
+        if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
+        return eps * tlog(tdiv(val + en, ((T)1.0 - val) + en), tabL);
+    };
+    // A7 + A8 + a(psi) at a cell: g* the undivided gradient of psi, pc0 psi
+    // and f phi at the cell.  The factor 1/4 (1 - tanh^2(psi/(2 eps))) of A9
+    // comes from the cell's phi, no tanh: psi = eps log(A/B), A = val + enum,
+    // B = (1 - val) + enum, so it is A B / (A + B)^2 exactly (and, like the
+    // tanh form, goes to 0 with A as phi nears -enum).  A NaN psi (phi below
+    // -enum, Q7) stays NaN, as tanh(NaN) would.
+    auto sharp = [&](T gx, T gy, T gz, T pc0, T f, T &sx, T &sy, T &sz) {
+        T normod = tdiv((T)1.0, tsqrt((gx * gx + gy * gy) + gz * gz) + tiny);
+        T val = f < (T)1.0 ? f : (T)1.0;
+        if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // as psi's FP32 clamp
+        const T A = val + en, B = ((T)1.0 - val) + en, S = A + B;
+        const T a = pc0 != pc0 ? pc0 : tdiv(A * B, S * S);
+        sx = a * (gx * normod);
+        sy = a * (gy * normod);
+        sz = a * (gz * normod);
     };
 
     if constexpr (sizeof(T) == 8) {
-        for (int q = tid; q < MATH_TAB; q += ATHREADS) tabL[q] = mtab[q];
+        for (int q = tid; q < NTAB; q += ATHREADS) tabL[q] = mtab[q];
         __syncthreads();
     }
 
-    // prologue: psi and phi for planes k0-2 .. k0+1, s for planes k0-1, k0.
-    // The psi ring has 3 slots, so s(k0-1) must read psi(k0-2) before
-    // psi(k0+1) takes its slot.
-    for (int m = k0 - 2; m <= k0; m++) {
-        fetch_phi(m);
-        psi_plane(m);
-        phi_store(m);
-    }
-    __syncthreads();
-    s_plane(k0 - 1);
-    fetch_phi(k0 + 1);
-    __syncthreads();
-    psi_plane(k0 + 1);
-    phi_store(k0 + 1);
-    fetch_phi(k0 + 2);
-    fetch_uvw(k0);
-    __syncthreads();
-    s_plane(k0);
-    __syncthreads();
+    // Column registers: phi of the own cell (f*) and of the ring cell (hf*)
+    // for the planes not yet in LDS, the z flux of the own cell (z*); suffix
+    // = plane - k.  psi(k), psi(k+1) and phi(k-1), phi(k) of a thread's own
+    // cells are read back from the LDS slots it wrote itself.
+    T f_p1, f_p2, fpre = 0;
+    T z_m1, z_0;
+    T hf_p1 = 0, hf_p2 = 0, hfpre = 0;
+    T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the output plane
+    const size_t xpo = (size_t)wrap(i + 1, n) + (size_t)n * wrap(j, n);
+    const size_t ypo = (size_t)wrap(i, n) + (size_t)n * wrap(j + 1, n);
+    auto fetch_uvw = [&](int m) {
+        if (active) {
+            const size_t z = nn * (size_t)m;
+            nu0 = u[own + z]; nuxp = u[xpo + z];
+            nv0 = v[own + z]; nvyp = v[ypo + z];
+            nw0 = w[own + z];
+        }
+    };
 
-    const int kend = min(k0 + kz, n);
-    const int sc0 = (ty + 1) * ASX + (tx + 1);
-    for (int k = k0; k < kend; k++) {
-        psi_plane(k + 2);
-        const T cu0 = nu0, cuxp = nuxp, cv0 = nv0, cvyp = nvyp, cw0 = nw0;
-        T wkp_raw = (T)0.0;
-        if (k + 1 < kend) {
-            fetch_uvw(k + 1);
-            wkp_raw = nw0;
-        } else if (active) {
-            wkp_raw = w[cxy + nn * (size_t)(k + 1 == n ? 0 : k + 1)];
+    // prologue: psi(k0-1), psi(k0) in LDS -> z flux of k0-1, s(k0), phi(k0-1),
+    // phi(k0) in LDS -> psi(k0+1) in LDS.
+    {
+        const T fm2 = ld_phi(own, k0 - 2), fm1 = ld_phi(own, k0 - 1), f0 = ld_phi(own, k0);
+        f_p1 = ld_phi(own, k0 + 1); f_p2 = ld_phi(own, k0 + 2);
+        T hfm1 = 0, hf0 = 0;
+        if (hk) {
+            hfm1 = ld_phi(hoff, k0 - 1); hf0 = ld_phi(hoff, k0); hf_p1 = ld_phi(hoff, k0 + 1);
+            hf_p2 = ld_phi(hoff, k0 + 2);
+        }
+        fetch_uvw(k0);
+        const T qm2 = psi_of(fm2), qm1 = psi_of(fm1), q0 = psi_of(f0), qp1 = psi_of(f_p1);
+        T hqm1 = 0, hq0 = 0, hqp1 = 0;
+        if (hk) { hqm1 = psi_of(hfm1); hq0 = psi_of(hf0); hqp1 = psi_of(hf_p1); }
+        const int sm1 = (k0 - 1) & 1, s0 = k0 & 1;
+        psiL[sm1][pI] = qm1; psiL[s0][pI] = q0;
+        if (hk) { psiL[sm1][hP] = hqm1; psiL[s0][hP] = hq0; }
+        __syncthreads();
+        T sx, sy;
+        sharp(psiL[sm1][pI + 1] - psiL[sm1][pI - 1], psiL[sm1][pI + APX] - psiL[sm1][pI - APX], q0 - qm2, qm1,
+              fm1, sx, sy, z_m1);
+        sharp(psiL[s0][pI + 1] - psiL[s0][pI - 1], psiL[s0][pI + APX] - psiL[s0][pI - APX], qp1 - qm1, q0, f0,
+              sx, sy, z_0);
+        sxL[s0][sI] = sx; syL[s0][sI] = sy;
+        phL[s0][sI] = f0; phL[sm1][sI] = fm1;
+        if (hk == 1 || hk == 2) {
+            T hx, hy, hz;
+            sharp(psiL[s0][hP + 1] - psiL[s0][hP - 1], psiL[s0][hP + APX] - psiL[s0][hP - APX], hqp1 - hqm1, hq0,
+                  hf0, hx, hy, hz);
+            if (hk == 1) sxL[s0][hS] = hx; else syL[s0][hS] = hy;
+            phL[s0][hS] = hf0;
         }
         __syncthreads();
-        phi_store(k + 2);
-        fetch_phi(k + 3);
-        s_plane(k + 1);
+        psiL[sm1][pI] = qp1;                 // the slot of plane k0+1
+        if (hk) psiL[sm1][hP] = hqp1;
         __syncthreads();
+    }
+
+    const int kend = min(k0 + kz, n);
+    for (int k = k0; k < kend; k++) {
+        const int s0 = k & 1, s1 = (k + 1) & 1;
+        // psi(k+2), and the phi loads of the next iteration
+        const T q_p2 = psi_of(f_p2);
+        T hq_p2 = 0;
+        if (hk) hq_p2 = psi_of(hf_p2);
+        fpre = ld_phi(own, k + 3);
+        if (hk) hfpre = ld_phi(hoff, k + 3);
+        // s(k+1): x/y parts of the normal from psi(k+1) in LDS, z part from the
+        // column (psi(k) from the own slot, psi(k+2) just formed)
+        T sx, sy, z_p1;
+        sharp(psiL[s1][pI + 1] - psiL[s1][pI - 1], psiL[s1][pI + APX] - psiL[s1][pI - APX], q_p2 - psiL[s0][pI],
+              psiL[s1][pI], f_p1, sx, sy, z_p1);
+        sxL[s1][sI] = sx; syL[s1][sI] = sy;
+        if (hk == 1 || hk == 2) {
+            T hx, hy, hz;
+            sharp(psiL[s1][hP + 1] - psiL[s1][hP - 1], psiL[s1][hP + APX] - psiL[s1][hP - APX],
+                  hq_p2 - psiL[s0][hP], psiL[s1][hP], hf_p1, hx, hy, hz);
+            if (hk == 1) sxL[s1][hS] = hx; else syL[s1][hS] = hy;
+        }
+        // the output plane k
         if (active) {
-            const int p0 = (k - k0 + 4) & 3, pm = (k - 1 - k0 + 4) & 3, pp = (k + 1 - k0 + 4) & 3;
-            T f0 = phL[p0][sc0];
-            T fxp = phL[p0][sc0 + 1], fxm = phL[p0][sc0 - 1];
-            T fyp = phL[p0][sc0 + ASX], fym = phL[p0][sc0 - ASX];
-            T fzp = phL[pp][sc0], fzm = phL[pm][sc0];
-            T uip = cuxp - um, u0 = cu0 - um;
-            T vjp = cvyp - vm, v0 = cv0 - vm;
-            T wkp = wkp_raw - wm, w0 = cw0 - wm;
+            const T f0 = phL[s0][sI];
+            T fxp = phL[s0][sI + 1], fxm = phL[s0][sI - 1];
+            T fyp = phL[s0][sI + ASX], fym = phL[s0][sI - ASX];
+            T fzp = f_p1, fzm = phL[s1][sI];
+            T wkp_raw;
+            if (k + 1 < kend) wkp_raw = w[own + nn * (size_t)(k + 1)];
+            else wkp_raw = w[own + nn * (size_t)(k + 1 == n ? 0 : k + 1)];
+            T uip = nuxp - um, u0 = nu0 - um;
+            T vjp = nvyp - vm, v0 = nv0 - vm;
+            T wkp = wkp_raw - wm, w0 = nw0 - wm;
             T A = uip * (T)0.5 * (fxp + f0) - u0 * (T)0.5 * (f0 + fxm);
             T B = vjp * (T)0.5 * (fyp + f0) - v0 * (T)0.5 * (f0 + fym);
             T C = wkp * (T)0.5 * (fzp + f0) - w0 * (T)0.5 * (f0 + fzm);
@@ -242,18 +269,28 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             T dyy = eps * ((fyp - (T)2.0 * f0) + fym) * ddxi;
             T dzz = eps * ((fzp - (T)2.0 * f0) + fzm) * ddxi;
             T rphi = r + gamma * ((dxx + dyy) + dzz);
-            const int sxy = (k - k0 + 2) & 1, sm = (k - 1 - k0 + 3) % 3, sp = (k + 1 - k0 + 3) % 3;
-            T X = (sxyL[sxy][0][sc0 + 1] - sxyL[sxy][0][sc0 - 1]) * (T)0.5 * dxi;
-            T Y = (sxyL[sxy][1][sc0 + ASX] - sxyL[sxy][1][sc0 - ASX]) * (T)0.5 * dxi;
-            T Z = (szL[sp][sc0] - szL[sm][sc0]) * (T)0.5 * dxi;
+            T X = (sxL[s0][sI + 1] - sxL[s0][sI - 1]) * (T)0.5 * dxi;
+            T Y = (syL[s0][sI + ASX] - syL[s0][sI - ASX]) * (T)0.5 * dxi;
+            T Z = (z_p1 - z_m1) * (T)0.5 * dxi;
             T r2 = rphi - gamma * ((X + Y) + Z);
-            __builtin_nontemporal_store(f0 + (T)c.dt * r2, &phin[cxy + nn * (size_t)k]);
+            __builtin_nontemporal_store(f0 + (T)c.dt * r2, &phin[own + nn * (size_t)k]);
         }
+        if (k + 1 < kend) fetch_uvw(k + 1);
+        // psi(k+2) and phi(k+1) for the x/y neighbours of the next iteration
+        psiL[s0][pI] = q_p2;
+        phL[s1][sI] = f_p1;
+        if (hk) psiL[s0][hP] = hq_p2;
+        if (hk == 1 || hk == 2) phL[s1][hS] = hf_p1;
+        __syncthreads();
+        f_p1 = f_p2; f_p2 = fpre;
+        z_m1 = z_0; z_0 = z_p1;
+        hf_p1 = hf_p2; hf_p2 = hfpre;
     }
 }
 
-// FP32: 8 waves per SIMD (64 VGPRs, a few spilled) measured 12% faster than
-// the 7 the register count would give; FP64 is held at 4 by its LDS rings.
+// FP32: 8 waves per SIMD (64 VGPRs).  FP64: 126 VGPRs, 4 waves per SIMD (two
+// blocks per CU); held to the 80 VGPRs of 6 waves (three blocks, which the
+// 44 KB of LDS would allow) the pass spills and runs 2.3x slower.
 template <typename T>
 __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
