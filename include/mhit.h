@@ -236,6 +236,17 @@ int mhit_set_tuning(mhit_ctx *ctx, const mhit_tuning *in);
  * context keeps until mhit_destroy; later calls allocate nothing. */
 int mhit_drop_count(mhit_ctx *ctx, int max_drops, int *n_drops, double *volume, double *deq);
 
+/* mhit_drop_count without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The labelling workspace is
+ * a static device array of MHIT_NOALLOC_MAX_N^3-cell size (11.6 MB), one per
+ * GPU, held until the process ends; boxes with N > MHIT_NOALLOC_MAX_N get
+ * MHIT_ERR_BAD_N.  Same arguments and results as mhit_drop_count.  Every call
+ * on one GPU uses that GPU's array: calls that overlap on one GPU (from
+ * several threads or streams) race. */
+#define MHIT_NOALLOC_MAX_N 128
+int mhit_drop_count_noalloc(mhit_ctx *ctx, int max_drops, int *n_drops, double *volume, double *deq);
+
 /* Diagnostics (no counterpart in the reference): evaluate the device's own
  * elementary functions on `count` arguments (op 0 log, 1 tanh, 2 sqrt,
  * 3 reciprocal), and copy an intermediate device buffer of the last step

@@ -25,6 +25,7 @@ MHIT_MIXED = 1
 MHIT_K_REFERENCE = 0
 MHIT_K_MODIFIED = 1
 MHIT_BATCH_MAX = 16
+MHIT_NOALLOC_MAX_N = 128
 
 # Names of the C-ABI entry points declared in include/mhit.h.
 EXPORTS = (
@@ -32,7 +33,7 @@ EXPORTS = (
     "mhit_destroy", "mhit_init_fields", "mhit_set_field", "mhit_get_field",
     "mhit_step", "mhit_step_batch", "mhit_reset_time", "mhit_init_random_velocity", "mhit_write_field", "mhit_read_field",
     "mhit_diagnostics", "mhit_kernel_timing", "mhit_kernel_stats", "mhit_device_bytes",
-    "mhit_debug_math", "mhit_debug_buffer", "mhit_autotune", "mhit_get_tuning", "mhit_set_tuning", "mhit_drop_count", "mhit_save_state", "mhit_load_state",
+    "mhit_debug_math", "mhit_debug_buffer", "mhit_autotune", "mhit_get_tuning", "mhit_set_tuning", "mhit_drop_count", "mhit_drop_count_noalloc", "mhit_save_state", "mhit_load_state",
     "mhit_ens_create", "mhit_ens_destroy", "mhit_ens_submit", "mhit_ens_progress", "mhit_ens_wait_idle",
     "mhit_ens_results", "mhit_ens_get_field", "mhit_ens_release", "mhit_ens_latency",
     "mhit_strerror", "mhit_abi_version",
@@ -142,6 +143,7 @@ def load():
         "mhit_autotune": ([vp, ctypes.c_int, P(Tuning)], ctypes.c_int),
         "mhit_get_tuning": ([vp, P(Tuning)], ctypes.c_int),
         "mhit_drop_count": ([vp, ctypes.c_int, P(ctypes.c_int), vp, vp], ctypes.c_int),
+        "mhit_drop_count_noalloc": ([vp, ctypes.c_int, P(ctypes.c_int), vp, vp], ctypes.c_int),
         "mhit_save_state": ([vp, ctypes.c_char_p], ctypes.c_int),
         "mhit_load_state": ([vp, ctypes.c_char_p], ctypes.c_int),
         "mhit_set_tuning": ([vp, P(Tuning)], ctypes.c_int),

@@ -159,6 +159,14 @@ constexpr size_t ccl_workspace_bytes(int n)
            ccl_up(ccl_cap((size_t)n * n * n) * 8) + 2 * ccl_up(CCL_CHUNKS * 4) + 256;
 }
 CclWork ccl_carve(void *base, int n);
+// The allocation-free drop count works in a static device array sized for
+// boxes up to NOALLOC_MAX_N^3 cells (11.6 MB per GPU).
+constexpr int NOALLOC_MAX_N = 128;
+constexpr size_t NOALLOC_SCRATCH_BYTES = ccl_workspace_bytes(NOALLOC_MAX_N);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+// Device address of that array on `device` (looked up once per device with
+// hipGetSymbolAddress, that device made current for the lookup); null on error.
+void *noalloc_scratch(int device);
 void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream_t s);
 // Label, list the roots in raster order, count each drop's cells.  Afterwards
 // *w.nroots holds the drop count; w.vol[0..count) the volumes when count <= cap.

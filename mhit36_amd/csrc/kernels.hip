@@ -8,6 +8,8 @@
 #include "internal.h"
 #include "devmath.h"
 
+#include <mutex>
+
 namespace mhit {
 
 #define IDX(i, j, k) ((size_t)(i) + (size_t)n * ((size_t)(j) + (size_t)n * (size_t)(k)))
@@ -2082,6 +2084,34 @@ void launch_ccl_label(const Fields &f, const Consts &c, unsigned *lab, hipStream
 __global__ void __launch_bounds__(BLK) k_fill_u32(unsigned *__restrict__ p, size_t count, unsigned v)
 {
     for (size_t x = (size_t)blockIdx.x * BLK + threadIdx.x; x < count; x += (size_t)gridDim.x * BLK) p[x] = v;
+}
+
+// Scratch of the allocation-free drop count: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel or copy (on the host it is a stand-in, not a device
+// address); it uses the address hipGetSymbolAddress gives for each device.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+void *noalloc_scratch(int device)
+{
+    static std::mutex mu;
+    static void *addr[NOALLOC_MAX_DEVICES] = {};
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+        if (cur != device && hipSetDevice(device) != hipSuccess) return nullptr;
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        addr[device] = ptr;
+    }
+    return addr[device];
 }
 
 CclWork ccl_carve(void *base, int n)

@@ -1331,6 +1331,21 @@ extern "C" int mhit_drop_count(mhit_ctx *c, int max_drops, int *n_drops, double 
     return drop_count_in(c, ccl_carve(c->ccl_ws, c->kc.n), max_drops, n_drops, volume, deq);
 }
 
+// The same count with the workspace in the library's static device scratch
+// (no hipMalloc, hipFree or allocating call): for callers that cannot
+// allocate.  Calls that overlap on one GPU share that scratch and race.
+static_assert(NOALLOC_MAX_N == MHIT_NOALLOC_MAX_N, "the scratch size the header states");
+
+extern "C" int mhit_drop_count_noalloc(mhit_ctx *c, int max_drops, int *n_drops, double *volume, double *deq)
+{
+    int rc = drop_count_args(c, max_drops, n_drops, volume, deq);
+    if (rc) return rc;
+    if (ccl_workspace_bytes(c->kc.n) > NOALLOC_SCRATCH_BYTES) return MHIT_ERR_BAD_N;
+    void *ptr = noalloc_scratch(c->device);
+    if (!ptr) return MHIT_ERR_HIP;
+    return drop_count_in(c, ccl_carve(ptr, c->kc.n), max_drops, n_drops, volume, deq);
+}
+
 extern "C" int mhit_debug_math(int op, size_t count, const double *x, double *y)
 {
     if ((!x || !y) && count) return MHIT_ERR_ARG;
