@@ -109,6 +109,19 @@ __device__ __forceinline__ int2 ring_p_cell(int h)
     return make_int2(h - 2 * ATY - ATX, ATY + 1);
 }
 
+// base[off] with a 32-bit byte offset (off * sizeof(T) < 2^32): the load takes
+// the uniform base in scalar registers and the offset in one vector register.
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T *at_off(T *base, unsigned off)
+{
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off * (unsigned)sizeof(T));
+}
+
 template <typename T>
 __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
@@ -135,17 +148,19 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const int tx = tid % ATX, ty = tid / ATX;
     const int i = i0 + tx, j = j0 + ty;
     const bool active = (i < n) && (j < n);
-    const size_t own = (size_t)wrap(i, n) + (size_t)n * wrap(j, n);   // ghost cells (i or j >= n) wrap
+    // in-plane offsets (< n^2): the plane's base address is uniform, so loads
+    // take a scalar base and a 32-bit vector offset
+    const unsigned own = (unsigned)wrap(i, n) + (unsigned)n * (unsigned)wrap(j, n);   // ghost cells wrap
     const int pI = (ty + 2) * APX + (tx + 2), sI = (ty + 1) * ASX + (tx + 1);
     // ring cell: kind 1 = s ring with s_x, 2 = s ring with s_y, 3 = psi only
     int hk = 0;
     int2 hc = make_int2(0, 0);
     if (tid >= HS0 && tid < HS0 + NHS) { hk = tid - HS0 < 2 * ATY ? 1 : 2; hc = ring_s_cell(tid - HS0); }
     else if (tid >= HP0 && tid < HP0 + NHP) { hk = 3; hc = ring_p_cell(tid - HP0); }
-    const size_t hoff = (size_t)wrap(i0 + hc.x, n) + (size_t)n * wrap(j0 + hc.y, n);
+    const unsigned hoff = (unsigned)wrap(i0 + hc.x, n) + (unsigned)n * (unsigned)wrap(j0 + hc.y, n);
     const int hP = (hc.y + 2) * APX + (hc.x + 2), hS = (hc.y + 1) * ASX + (hc.x + 1);
 
-    auto ld_phi = [&](size_t off, int m) { return phi[off + nn * (size_t)wrap(m, n)]; };
+    auto ld_phi = [&](unsigned off, int m) { return ld_off(phi + nn * (size_t)wrap(m, n), off); };
     auto psi_of = [&](T f) {                  // A6
         T val = f < (T)1.0 ? f : (T)1.0;
         if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
@@ -181,14 +196,20 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     T z_m1, z_0;
     T hf_p1 = 0, hf_p2 = 0, hfpre = 0;
     T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the output plane
-    const size_t xpo = (size_t)wrap(i + 1, n) + (size_t)n * wrap(j, n);
-    const size_t ypo = (size_t)wrap(i, n) + (size_t)n * wrap(j + 1, n);
+    // the x+1 and y+1 neighbours of the own cell, as offsets from it (periodic)
+    const unsigned xpo = (unsigned)wrap(i + 1, n) + (unsigned)n * (unsigned)wrap(j, n);
+    const unsigned ypo = (unsigned)wrap(i, n) + (unsigned)n * (unsigned)wrap(j + 1, n);
     auto fetch_uvw = [&](int m) {
         if (active) {
-            const size_t z = nn * (size_t)m;
-            nu0 = u[own + z]; nuxp = u[xpo + z];
-            nv0 = v[own + z]; nvyp = v[ypo + z];
-            nw0 = w[own + z];
+            // the plane offset is opaque: otherwise the compiler hoists
+            // base + own out of the march as five 64-bit vector pointers,
+            // which spill
+            size_t z = nn * (size_t)m;
+            asm volatile("" : "+s"(z));
+            const T *up = u + z, *vp = v + z, *wp = w + z;
+            nu0 = ld_off(up, own); nuxp = ld_off(up, xpo);
+            nv0 = ld_off(vp, own); nvyp = ld_off(vp, ypo);
+            nw0 = ld_off(wp, own);
         }
     };
 
@@ -258,24 +279,39 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             T fyp = phL[s0][sI + ASX], fym = phL[s0][sI - ASX];
             T fzp = f_p1, fzm = phL[s1][sI];
             T wkp_raw;
-            if (k + 1 < kend) wkp_raw = w[own + nn * (size_t)(k + 1)];
-            else wkp_raw = w[own + nn * (size_t)(k + 1 == n ? 0 : k + 1)];
+            wkp_raw = ld_off(w + nn * (size_t)(k + 1 == n ? 0 : k + 1), own);
             T uip = nuxp - um, u0 = nu0 - um;
             T vjp = nvyp - vm, v0 = nv0 - vm;
             T wkp = wkp_raw - wm, w0 = nw0 - wm;
-            T A = uip * (T)0.5 * (fxp + f0) - u0 * (T)0.5 * (f0 + fxm);
-            T B = vjp * (T)0.5 * (fyp + f0) - v0 * (T)0.5 * (f0 + fym);
-            T C = wkp * (T)0.5 * (fzp + f0) - w0 * (T)0.5 * (f0 + fzm);
-            T r = (-(A * dxi) - B * dxi) - C * dxi;
-            T dxx = eps * ((fxp - (T)2.0 * f0) + fxm) * ddxi;
-            T dyy = eps * ((fyp - (T)2.0 * f0) + fym) * ddxi;
-            T dzz = eps * ((fzp - (T)2.0 * f0) + fzm) * ddxi;
-            T rphi = r + gamma * ((dxx + dyy) + dzz);
-            T X = (sxL[s0][sI + 1] - sxL[s0][sI - 1]) * (T)0.5 * dxi;
-            T Y = (syL[s0][sI + ASX] - syL[s0][sI - ASX]) * (T)0.5 * dxi;
-            T Z = (z_p1 - z_m1) * (T)0.5 * dxi;
-            T r2 = rphi - gamma * ((X + Y) + Z);
-            __builtin_nontemporal_store(f0 + (T)c.dt * r2, &phin[own + nn * (size_t)k]);
+            if constexpr (sizeof(T) == 4) {
+                // FP32 (mixed precision, held to the FP64 result by tolerance):
+                // the same discretisation with the constant factors gathered
+                // and fused multiply-adds, about half the instructions of the
+                // operand-by-operand form below.
+                const T hd = (T)0.5 * dxi;
+                const T A = __builtin_fmaf(uip, fxp + f0, -(u0 * (f0 + fxm)));
+                const T B = __builtin_fmaf(vjp, fyp + f0, -(v0 * (f0 + fym)));
+                const T C = __builtin_fmaf(wkp, fzp + f0, -(w0 * (f0 + fzm)));
+                const T lap = ((fxp + fxm) + (fyp + fym)) + __builtin_fmaf((T)-6.0, f0, fzp + fzm);
+                const T sdiv = ((sxL[s0][sI + 1] - sxL[s0][sI - 1]) + (syL[s0][sI + ASX] - syL[s0][sI - ASX])) +
+                               (z_p1 - z_m1);
+                const T r2 = __builtin_fmaf(gamma * eps * ddxi, lap, -hd * (((A + B) + C) + gamma * sdiv));
+                __builtin_nontemporal_store(__builtin_fmaf((T)c.dt, r2, f0), at_off(phin + nn * (size_t)k, own));
+            } else {
+                T A = uip * (T)0.5 * (fxp + f0) - u0 * (T)0.5 * (f0 + fxm);
+                T B = vjp * (T)0.5 * (fyp + f0) - v0 * (T)0.5 * (f0 + fym);
+                T C = wkp * (T)0.5 * (fzp + f0) - w0 * (T)0.5 * (f0 + fzm);
+                T r = (-(A * dxi) - B * dxi) - C * dxi;
+                T dxx = eps * ((fxp - (T)2.0 * f0) + fxm) * ddxi;
+                T dyy = eps * ((fyp - (T)2.0 * f0) + fym) * ddxi;
+                T dzz = eps * ((fzp - (T)2.0 * f0) + fzm) * ddxi;
+                T rphi = r + gamma * ((dxx + dyy) + dzz);
+                T X = (sxL[s0][sI + 1] - sxL[s0][sI - 1]) * (T)0.5 * dxi;
+                T Y = (syL[s0][sI + ASX] - syL[s0][sI - ASX]) * (T)0.5 * dxi;
+                T Z = (z_p1 - z_m1) * (T)0.5 * dxi;
+                T r2 = rphi - gamma * ((X + Y) + Z);
+                __builtin_nontemporal_store(f0 + (T)c.dt * r2, at_off(phin + nn * (size_t)k, own));
+            }
         }
         if (k + 1 < kend) fetch_uvw(k + 1);
         // psi(k+2) and phi(k+1) for the x/y neighbours of the next iteration
@@ -1191,6 +1227,24 @@ __device__ __forceinline__ void nt_store2(double2 *dst, double re, double im)
     __builtin_nontemporal_store(v, reinterpret_cast<nt_double2 *>(dst));
 }
 
+// The value `x` of lane `src` of this wavefront: a cross-lane permute
+// (ds_bpermute over the LDS crossbar; no LDS allocation, no barrier).
+__device__ __forceinline__ double2 lane_get2(double2 x, int src)
+{
+    return make_double2(__shfl(x.x, src, 64), __shfl(x.y, src, 64));
+}
+
+// The x passes pair position q of a line with its Hermitian mirror n - q.  A
+// line of n <= 512 points sits in LM = n/8 consecutive lanes of one
+// wavefront, thread tj holding positions tj + m*LM; for tj > 0 the mirror of
+// tj + m*LM is (LM - tj) + (7 - m)*LM, held by lane LM - tj of the line in
+// slot 7 - m, and for tj = 0 the mirror of m*LM is (8 - m)*LM, the thread's
+// own slot 8 - m.  So the mirror comes from one permute of slot 7 - m.
+__device__ __forceinline__ int mirror_lane(int tj, int LM)
+{
+    return (int)(threadIdx.x & 63) - tj + ((LM - tj) & (LM - 1));
+}
+
 template <typename T>
 __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
@@ -1226,6 +1280,27 @@ __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
         v[m] = make_double2(rr[0], rr[1]);
     }
     fft_regs<-1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
+    if (LM <= 64) {
+        // split z into the two real lines' half spectra (q = 0 .. n/2); z[n-q]
+        // from the mirror lane instead of a round trip through LDS
+        const int src = mirror_lane(tj, LM);
+        double2 *r1 = spec + spec_row(k, j, n, ld), *r2 = spec + spec_row(k, j + 1, n, ld);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int q = tj + m * LM;
+            double2 zm = lane_get2(v[7 - m], src);
+            if (tj == 0) zm = v[(8 - m) & 7];
+            const double a = v[m].x, b = v[m].y, cc = zm.x, d = zm.y;
+            nt_store2(&r1[q], (a + cc) * 0.5, (b - d) * 0.5);
+            nt_store2(&r2[q], (b + d) * 0.5, (cc - a) * 0.5);
+        }
+        if (tj == 0) {                        // q = n/2, its own mirror
+            const double a = v[4].x, b = v[4].y, cc = v[4].x, d = v[4].y;
+            nt_store2(&r1[half], (a + cc) * 0.5, (b - d) * 0.5);
+            nt_store2(&r2[half], (b + d) * 0.5, (cc - a) * 0.5);
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < 8; m++) lds[rlds<true>(pl_, tj + m * LM, lstride, 1)] = v[m];
     __syncthreads();
@@ -1368,21 +1443,47 @@ __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restri
     const double2 *s1 = spec + spec_row(k, j, n, ld);
     const double2 *s2 = spec + spec_row(k, j + 1, n, ld);
     double2 v[8];
+    if (LM <= 64) {
+        // Each spectrum value is loaded once: positions below n/2 directly,
+        // their mirrors (slots 4..7) from the mirror lane by a permute.
+        double2 A[4], B[4];
 #pragma unroll
-    for (int m = 0; m < 8; m++) {
-        const int q = tj + m * LM;
-        double2 z;
-        if (q == 0 || q == half) {
-            z = make_double2(s1[q].x, s2[q].x);
-        } else if (q < half) {
-            double2 a = s1[q], b = s2[q];
-            z = make_double2(a.x - b.y, a.y + b.x);
-        } else {
-            int mm = n - q;
-            double2 a = s1[mm], b = s2[mm];
-            z = make_double2(a.x + b.y, b.x - a.y);
+        for (int m = 0; m < 4; m++) {
+            A[m] = s1[tj + m * LM];
+            B[m] = s2[tj + m * LM];
         }
-        v[m] = z;
+        const int src = mirror_lane(tj, LM);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const double2 a = A[m], b = B[m];
+            v[m] = tj + m * LM == 0 ? make_double2(a.x, b.x) : make_double2(a.x - b.y, a.y + b.x);
+        }
+#pragma unroll
+        for (int m = 4; m < 8; m++) {
+            double2 a = lane_get2(A[7 - m], src), b = lane_get2(B[7 - m], src);
+            if (tj == 0) {
+                if (m == 4) { a = s1[half]; b = s2[half]; }
+                else { a = A[8 - m]; b = B[8 - m]; }
+            }
+            v[m] = (tj == 0 && m == 4) ? make_double2(a.x, b.x) : make_double2(a.x + b.y, b.x - a.y);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int q = tj + m * LM;
+            double2 z;
+            if (q == 0 || q == half) {
+                z = make_double2(s1[q].x, s2[q].x);
+            } else if (q < half) {
+                double2 a = s1[q], b = s2[q];
+                z = make_double2(a.x - b.y, a.y + b.x);
+            } else {
+                int mm = n - q;
+                double2 a = s1[mm], b = s2[mm];
+                z = make_double2(a.x + b.y, b.x - a.y);
+            }
+            v[m] = z;
+        }
     }
     fft_regs<1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
 #pragma unroll
