@@ -9,6 +9,7 @@
 #include "devmath.h"
 
 #include <mutex>
+#include <type_traits>
 
 namespace mhit {
 
@@ -1217,6 +1218,61 @@ __device__ void fft_regs(double2 (&v)[8], int tj, int line, double2 *lds, int ls
     }
 }
 
+// The stage plan of a power-of-two length n = 2^L at compile time: the
+// radices, sub-lengths Ns and twiddle offsets that the host plan builder
+// gives for that n (radix-8 stages, then two 4s when L mod 3 = 1, one 4 when
+// L mod 3 = 2).  With L known the stage loop unrolls and every index of the
+// exchanges (jj / Ns, jj % Ns, the line strides) is a shift or a mask; the
+// generic fft_regs computes them from the runtime plan with integer divides.
+__host__ __device__ constexpr int cplan_nst(int L)
+{
+    return L % 3 == 0 ? L / 3 : L % 3 == 2 ? L / 3 + 1 : L / 3 + 1;
+}
+__host__ __device__ constexpr int cplan_R(int L, int s)
+{
+    return L % 3 == 0 ? 8 : L % 3 == 2 ? (s < L / 3 ? 8 : 4) : (s < L / 3 - 1 ? 8 : 4);
+}
+__host__ __device__ constexpr int cplan_Ns(int L, int s)
+{
+    int ns = 1;
+    for (int t = 0; t < s; t++) ns *= cplan_R(L, t);
+    return ns;
+}
+__host__ __device__ constexpr int cplan_twoff(int L, int s)
+{
+    int o = 0;
+    for (int t = 0; t < s; t++) o += cplan_Ns(L, t) * (cplan_R(L, t) - 1);
+    return o;
+}
+
+template <int SG, bool XL, int L, int S = 0>
+__device__ __forceinline__ void fft_regs_c(double2 (&v)[8], int tj, int line, double2 *lds, int lstride,
+                                           int pstride, const double2 *__restrict__ tw)
+{
+    constexpr int LM = (1 << L) / 8, R = cplan_R(L, S), Ns = cplan_Ns(L, S);
+    if constexpr (S > 0) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = lds[rlds<XL>(line, tj + m * LM, lstride, pstride)];
+        __syncthreads();
+    }
+    stage_regs<R, SG>(v, tj, LM, Ns, tw + cplan_twoff(L, S), S == 0);
+    if constexpr (S + 1 < cplan_nst(L)) {
+        store_stage<R, XL>(v, lds, line, tj, LM, Ns, lstride, pstride);
+        __syncthreads();
+        fft_regs_c<SG, XL, L, S + 1>(v, tj, line, lds, lstride, pstride, tw);
+    }
+}
+
+// All stages of one line: the compile-time plan for n = 2^L, L > 0, else the
+// runtime plan pl.
+template <int SG, bool XL, int L>
+__device__ __forceinline__ void fft_lines(double2 (&v)[8], int tj, int line, double2 *lds, int lstride,
+                                          int pstride, const FftPlan &pl, const double2 *__restrict__ tw)
+{
+    if constexpr (L > 0) fft_regs_c<SG, XL, L>(v, tj, line, lds, lstride, pstride, tw);
+    else fft_regs<SG, XL>(v, tj, line, lds, lstride, pstride, pl, tw);
+}
+
 __host__ __device__ constexpr int rline_stride(int n) { return n + n / 8 + 1; }
 
 // 16-byte streaming store of one complex value (written once, not re-read by this kernel).
@@ -1245,7 +1301,7 @@ __device__ __forceinline__ int mirror_lane(int tj, int LM)
     return (int)(threadIdx.x & 63) - tj + ((LM - tj) & (LM - 1));
 }
 
-template <typename T>
+template <typename T, int L = 0>
 __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
@@ -1253,7 +1309,7 @@ __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
-    const int n = c.n, half = n / 2, nh = n / 2 + 1, LM = n / 8;
+    const int n = L ? (1 << L) : c.n, half = n / 2, nh = n / 2 + 1, LM = n / 8;
     const int lstride = rline_stride(n);
     const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
     const int pair = blk.x * ppb + pl_;
@@ -1279,7 +1335,7 @@ __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
         }
         v[m] = make_double2(rr[0], rr[1]);
     }
-    fft_regs<-1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
+    fft_lines<-1, true, L>(v, tj, pl_, lds, lstride, 1, pl, tw);
     if (LM <= 64) {
         // split z into the two real lines' half spectra (q = 0 .. n/2); z[n-q]
         // from the mirror lane instead of a round trip through LDS
@@ -1316,25 +1372,26 @@ __device__ __forceinline__ void k_xfwd_r_body(uint3 blk,
     }
 }
 
-template <typename T>
+template <typename T, int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xfwd_r(
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     double2 *__restrict__ spec, const double2 *__restrict__ tw, const Scalars *__restrict__ sc,
     Consts c, FftPlan pl, int ppb, int ld)
 {
-    k_xfwd_r_body<T>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, spec, tw, sc, c, pl, ppb, ld);
+    k_xfwd_r_body<T, L>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, spec, tw, sc, c, pl, ppb, ld);
 }
 
 
-template <int SG>
+// L > 0: n = 2^L and TK columns per block, both compile-time
+template <int SG, int L = 0, int TK = 0>
 __device__ __forceinline__ void k_ypass_r_body(uint3 blk, double2 *__restrict__ spec,
                                                              const double2 *__restrict__ tw,
                                                              const Scalars *__restrict__ sc, FftPlan pl, int ld,
-                                                             int tkx)
+                                                             int tkx_)
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
-    const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
+    const int n = L ? (1 << L) : pl.n, nh = n / 2 + 1, LM = n / 8, tkx = TK ? TK : tkx_;
     const int q0 = blk.x * tkx, k = blk.y;
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const bool ok = q0 + line < nh;
@@ -1342,33 +1399,33 @@ __device__ __forceinline__ void k_ypass_r_body(uint3 blk, double2 *__restrict__ 
     double2 v[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * ld] : make_double2(0.0, 0.0);
-    fft_regs<SG, false>(v, tj, line, lds, 1, tkx, pl, tw);
+    fft_lines<SG, false, L>(v, tj, line, lds, 1, tkx, pl, tw);
     if (ok) {
 #pragma unroll
         for (int m = 0; m < 8; m++) nt_store2(&base[(size_t)(tj + m * LM) * ld], v[m].x, v[m].y);
     }
 }
 
-template <int SG>
+template <int SG, int L, int TK>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass_r(double2 *__restrict__ spec,
                                                              const double2 *__restrict__ tw,
                                                              const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                              int tkx)
 {
-    k_ypass_r_body<SG>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, sc, pl, ld, tkx);
+    k_ypass_r_body<SG, L, TK>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, sc, pl, ld, tkx);
 }
 
 
-template <bool INV_ONLY = false>
+template <bool INV_ONLY = false, int L = 0, int TK = 0>
 __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict__ spec,
                                                                 const double2 *__restrict__ tw,
                                                                 const double *__restrict__ ktab,
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
-                                                                int tkx)
+                                                                int tkx_)
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
-    const int n = pl.n, nh = n / 2 + 1, LM = n / 8;
+    const int n = L ? (1 << L) : pl.n, nh = n / 2 + 1, LM = n / 8, tkx = TK ? TK : tkx_;
     const int q0 = blk.x * tkx, j = blk.y;
     const int t = threadIdx.x, line = t % tkx, tj = t / tkx;
     const int q = q0 + line;
@@ -1379,7 +1436,7 @@ __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict
 #pragma unroll
     for (int m = 0; m < 8; m++) v[m] = ok ? base[(size_t)(tj + m * LM) * rs] : make_double2(0.0, 0.0);
     if constexpr (!INV_ONLY) {
-        fft_regs<-1, false>(v, tj, line, lds, 1, tkx, pl, tw);
+        fft_lines<-1, false, L>(v, tj, line, lds, 1, tkx, pl, tw);
         const double nnn = (double)((long long)n * (long long)n * (long long)n);
         const double ky = ktab[j], kx = ktab[ok ? q : 0];
 #pragma unroll
@@ -1393,7 +1450,7 @@ __device__ __forceinline__ void k_zpoisson_r_body(uint3 blk, double2 *__restrict
             v[m] = z;
         }
     }
-    fft_regs<1, false>(v, tj, line, lds, 1, tkx, pl, tw);
+    fft_lines<1, false, L>(v, tj, line, lds, 1, tkx, pl, tw);
     if (ok) {
 #pragma unroll
         for (int m = 0; m < 8; m++) nt_store2(&base[(size_t)(tj + m * LM) * rs], v[m].x, v[m].y);
@@ -1418,16 +1475,18 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zinv(double2 *__restrict__ 
     k_zpoisson_body<true>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
 }
 
+template <int L, int TK>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restrict__ spec,
                                                                 const double2 *__restrict__ tw,
                                                                 const double *__restrict__ ktab,
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                                 int tkx)
 {
-    k_zpoisson_r_body<false>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+    k_zpoisson_r_body<false, L, TK>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
 }
 
 
+template <int L = 0>
 __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restrict__ spec,
                                                             double *__restrict__ p, const double2 *__restrict__ tw,
                                                             const Scalars *__restrict__ sc, FftPlan pl, int ppb,
@@ -1435,7 +1494,7 @@ __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restri
 {
     if (halted(sc)) return;
     extern __shared__ double2 lds[];
-    const int n = pl.n, half = n / 2, LM = n / 8;
+    const int n = L ? (1 << L) : pl.n, half = n / 2, LM = n / 8;
     const int lstride = rline_stride(n);
     const int t = threadIdx.x, pl_ = t / LM, tj = t - pl_ * LM;
     const int pair = blk.x * ppb + pl_;
@@ -1485,7 +1544,7 @@ __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restri
             v[m] = z;
         }
     }
-    fft_regs<1, true>(v, tj, pl_, lds, lstride, 1, pl, tw);
+    fft_lines<1, true, L>(v, tj, pl_, lds, lstride, 1, pl, tw);
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int i = tj + m * LM;
@@ -1494,12 +1553,13 @@ __device__ __forceinline__ void k_xinv_r_body(uint3 blk, const double2 *__restri
     }
 }
 
+template <int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__restrict__ spec,
                                                             double *__restrict__ p, const double2 *__restrict__ tw,
                                                             const Scalars *__restrict__ sc, FftPlan pl, int ppb,
                                                             int ld)
 {
-    k_xinv_r_body(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
+    k_xinv_r_body<L>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
 }
 
 
@@ -2351,22 +2411,67 @@ static bool plan_regs(const FftPlan &pl)
 }
 
 // x-line pairs per block (register path): threads = p*n/8 <= 1024, LDS <= LDS_CAP.
-static int rpairs(const FftPlan &pl)
+static constexpr int rpairs_n(int n)
 {
     int p = 8;
-    while (p > 1 && p * (pl.n / 8) > FFT_MAX_THREADS) p >>= 1;
-    while (p > 1 && (size_t)p * rline_stride(pl.n) * sizeof(double2) > LDS_CAP) p >>= 1;
-    while (p > 1 && (pl.n * pl.n / 2) % p != 0) p >>= 1;
+    while (p > 1 && p * (n / 8) > FFT_MAX_THREADS) p >>= 1;
+    while (p > 1 && (size_t)p * rline_stride(n) * sizeof(double2) > LDS_CAP) p >>= 1;
+    while (p > 1 && (n * n / 2) % p != 0) p >>= 1;
     return p;
 }
+static int rpairs(const FftPlan &pl) { return rpairs_n(pl.n); }
 
 // kx columns per block (register path).
-static int rtile(const FftPlan &pl)
+static constexpr int rtile_n(int n)
 {
     int t = 8;
-    while (t > 1 && t * (pl.n / 8) > FFT_MAX_THREADS) t >>= 1;
-    while (t > 1 && (size_t)t * pl.n * sizeof(double2) > LDS_CAP) t >>= 1;
+    while (t > 1 && t * (n / 8) > FFT_MAX_THREADS) t >>= 1;
+    while (t > 1 && (size_t)t * n * sizeof(double2) > LDS_CAP) t >>= 1;
     return t;
+}
+static int rtile(const FftPlan &pl) { return rtile_n(pl.n); }
+
+// The register kernels are compiled for n = 2^L, 7 <= L <= 10, at their
+// default launch shapes (rtile, rpairs), with the stage plan and the column
+// tile as constants; other n and shapes (a tuned shape, say) run the kernels
+// compiled for any n, which read the plan at run time.  reg_log2 gives L when
+// pl is the compiled plan, else 0.
+constexpr int REG_LMIN = 7, REG_LMAX = 10;
+static constexpr int rtile_c(int L) { return rtile_n(1 << L); }
+static constexpr int rpairs_c(int L) { return rpairs_n(1 << L); }
+static int reg_log2(const FftPlan &pl)
+{
+    int L = 0;
+    while (L < 30 && (1 << L) < pl.n) L++;
+    if ((1 << L) != pl.n || L < REG_LMIN || L > REG_LMAX || pl.nst != cplan_nst(L)) return 0;
+    for (int st = 0; st < pl.nst; st++)
+        if (pl.R[st] != cplan_R(L, st) || pl.Ns[st] != cplan_Ns(L, st) || pl.twoff[st] != cplan_twoff(L, st))
+            return 0;
+    return L;
+}
+// Calls fn(std::integral_constant<int, L>) for L in [REG_LMIN, REG_LMAX], or
+// with 0 for any other L.
+template <typename Fn>
+static void with_log2(int L, Fn &&fn)
+{
+    switch (L) {
+    case 7: fn(std::integral_constant<int, 7>()); break;
+    case 8: fn(std::integral_constant<int, 8>()); break;
+    case 9: fn(std::integral_constant<int, 9>()); break;
+    case 10: fn(std::integral_constant<int, 10>()); break;
+    default: fn(std::integral_constant<int, 0>()); break;
+    }
+}
+// L for a column pass of tile tkx and for an x pass of ppb line pairs per block.
+static int reg_log2_cols(const FftPlan &pl, int tkx)
+{
+    const int L = reg_log2(pl);
+    return L && tkx == rtile_c(L) ? L : 0;
+}
+static int reg_log2_rows(const FftPlan &pl, int ppb)
+{
+    const int L = reg_log2(pl);
+    return L && ppb == rpairs_c(L) ? L : 0;
 }
 
 static int fft_threads(const FftPlan &pl, int nlines)
@@ -2420,12 +2525,15 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
         const int ppb = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
         const dim3 grid(pairs / ppb), blk(ppb * (c.n / 8));
-        if (c.mixed)
-            hipLaunchKernelGGL(k_xfwd_r<float>, grid, blk, shm, s, (const float *)f.us, (const float *)f.vs,
-                               (const float *)f.ws, f.spec, f.tw, f.sc, c, pl, ppb, f.ld);
-        else
-            hipLaunchKernelGGL(k_xfwd_r<double>, grid, blk, shm, s, f.us, f.vs, f.ws, f.spec, f.tw, f.sc, c, pl,
-                               ppb, f.ld);
+        with_log2(reg_log2_rows(pl, ppb), [&](auto LC) {
+            constexpr int L = decltype(LC)::value;
+            if (c.mixed)
+                hipLaunchKernelGGL((k_xfwd_r<float, L>), grid, blk, shm, s, (const float *)f.us,
+                                   (const float *)f.vs, (const float *)f.ws, f.spec, f.tw, f.sc, c, pl, ppb, f.ld);
+            else
+                hipLaunchKernelGGL((k_xfwd_r<double, L>), grid, blk, shm, s, f.us, f.vs, f.ws, f.spec, f.tw, f.sc, c,
+                                   pl, ppb, f.ld);
+        });
         li->name = "xfwd_div"; li->bytes = (c.mixed ? 2.5 : 4.0) * F(c);
         return;
     }
@@ -2448,10 +2556,13 @@ void launch_ypass(const Fields &f, const Consts &c, const FftPlan &pl, const Tun
         const int tkx = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl), nh = c.n / 2 + 1;
         const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
         const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
-        if (inverse)
-            hipLaunchKernelGGL(k_ypass_r<1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
-        else
-            hipLaunchKernelGGL(k_ypass_r<-1>, grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        with_log2(reg_log2_cols(pl, tkx), [&](auto LC) {
+            constexpr int L = decltype(LC)::value, TK = L ? rtile_c(L) : 0;
+            if (inverse)
+                hipLaunchKernelGGL((k_ypass_r<1, L, TK>), grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+            else
+                hipLaunchKernelGGL((k_ypass_r<-1, L, TK>), grid, blk, shm, s, f.spec, f.tw, f.sc, pl, f.ld, tkx);
+        });
         li->name = inverse ? "y_inv" : "y_fwd";
         li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
         return;
@@ -2476,7 +2587,10 @@ void launch_zpoisson(const Fields &f, const Consts &c, const FftPlan &pl, const 
         const int tkx = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl), nh = c.n / 2 + 1;
         const dim3 grid((nh + tkx - 1) / tkx, c.n), blk(tkx * (c.n / 8));
         const size_t shm = sizeof(double2) * (size_t)tkx * c.n;
-        hipLaunchKernelGGL(k_zpoisson_r, grid, blk, shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+        with_log2(reg_log2_cols(pl, tkx), [&](auto LC) {
+            constexpr int L = decltype(LC)::value, TK = L ? rtile_c(L) : 0;
+            hipLaunchKernelGGL((k_zpoisson_r<L, TK>), grid, blk, shm, s, f.spec, f.tw, f.ktab, f.sc, pl, f.ld, tkx);
+        });
         li->name = "z_poisson";
         li->bytes = 2.0 * 16.0 * (double)nh * c.n * c.n;
         return;
@@ -2497,8 +2611,10 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
     if (plan_regs(pl)) {
         const int ppb = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
-        hipLaunchKernelGGL(k_xinv_r, dim3(pairs / ppb), dim3(ppb * (c.n / 8)), shm, s, f.spec, f.p, f.tw, f.sc, pl,
-                           ppb, f.ld);
+        with_log2(reg_log2_rows(pl, ppb), [&](auto LC) {
+            hipLaunchKernelGGL((k_xinv_r<decltype(LC)::value>), dim3(pairs / ppb), dim3(ppb * (c.n / 8)), shm, s,
+                               f.spec, f.p, f.tw, f.sc, pl, ppb, f.ld);
+        });
         li->name = "x_inv"; li->bytes = 16.0 * (double)(c.n / 2 + 1) * c.n * c.n + F(c);
         return;
     }
