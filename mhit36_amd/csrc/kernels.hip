@@ -1275,6 +1275,25 @@ __device__ __forceinline__ void fft_lines(double2 (&v)[8], int tj, int line, dou
 
 __host__ __device__ constexpr int rline_stride(int n) { return n + n / 8 + 1; }
 
+// Default launch shapes of the register kernels: columns per block of the y/z
+// passes and x-line pairs per block (threads = shape * n/8 <= 1024, LDS within
+// LDS_CAP, pairs dividing n*n/2).
+static constexpr int rtile_n(int n)
+{
+    int t = 8;
+    while (t > 1 && t * (n / 8) > FFT_MAX_THREADS) t >>= 1;
+    while (t > 1 && (size_t)t * n * sizeof(double2) > LDS_CAP) t >>= 1;
+    return t;
+}
+static constexpr int rpairs_n(int n)
+{
+    int p = 8;
+    while (p > 1 && p * (n / 8) > FFT_MAX_THREADS) p >>= 1;
+    while (p > 1 && (size_t)p * rline_stride(n) * sizeof(double2) > LDS_CAP) p >>= 1;
+    while (p > 1 && (n * n / 2) % p != 0) p >>= 1;
+    return p;
+}
+
 // 16-byte streaming store of one complex value (written once, not re-read by this kernel).
 typedef double nt_double2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void nt_store2(double2 *dst, double re, double im)
@@ -1795,49 +1814,49 @@ __global__ void __launch_bounds__(PTHREADS) kg_momentum_x2(const GangBox *__rest
 }
 
 // FFT passes: box = the first grid dimension the one-box launch leaves unused.
-template <typename T, bool REGS>
+template <typename T, bool REGS, int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_xfwd(const GangBox *__restrict__ g, FftPlan pl, int ppb, int ld)
 {
     const GangBox &B = g[blockIdx.y];
     const uint3 b = make_uint3(blockIdx.x, 0, 0);
     if constexpr (REGS)
-        k_xfwd_r_body<T>(b, (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.spec, B.f.tw, B.f.sc, B.c,
+        k_xfwd_r_body<T, L>(b, (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.spec, B.f.tw, B.f.sc, B.c,
                          pl, ppb, ld);
     else
         k_xfwd_body<T>(b, (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.spec, B.f.tw, B.f.sc, B.c, pl,
                        ppb, ld);
 }
 
-template <int SG, bool REGS>
+template <int SG, bool REGS, int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_ypass(const GangBox *__restrict__ g, FftPlan pl, int ld, int tkx)
 {
     const GangBox &B = g[blockIdx.z];
     const uint3 b = make_uint3(blockIdx.x, blockIdx.y, 0);
     if constexpr (REGS)
-        k_ypass_r_body<SG>(b, B.f.spec, B.f.tw, B.f.sc, pl, ld, tkx);
+        k_ypass_r_body<SG, L, L ? rtile_n(1 << L) : 0>(b, B.f.spec, B.f.tw, B.f.sc, pl, ld, tkx);
     else
         k_ypass_body<SG>(b, B.f.spec, B.f.tw, B.f.sc, pl, ld, tkx);
 }
 
-template <bool REGS>
+template <bool REGS, int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_zpoisson(const GangBox *__restrict__ g, FftPlan pl, int ld,
                                                                int tkx)
 {
     const GangBox &B = g[blockIdx.z];
     const uint3 b = make_uint3(blockIdx.x, blockIdx.y, 0);
     if constexpr (REGS)
-        k_zpoisson_r_body<false>(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
+        k_zpoisson_r_body<false, L, L ? rtile_n(1 << L) : 0>(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
     else
         k_zpoisson_body<false>(b, B.f.spec, B.f.tw, B.f.ktab, B.f.sc, pl, ld, tkx);
 }
 
-template <bool REGS>
+template <bool REGS, int L>
 __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_xinv(const GangBox *__restrict__ g, FftPlan pl, int ppb, int ld)
 {
     const GangBox &B = g[blockIdx.y];
     const uint3 b = make_uint3(blockIdx.x, 0, 0);
     if constexpr (REGS)
-        k_xinv_r_body(b, B.f.spec, B.f.p, B.f.tw, B.f.sc, pl, ppb, ld);
+        k_xinv_r_body<L>(b, B.f.spec, B.f.p, B.f.tw, B.f.sc, pl, ppb, ld);
     else
         k_xinv_body(b, B.f.spec, B.f.p, B.f.tw, B.f.sc, pl, ppb, ld);
 }
@@ -2411,34 +2430,18 @@ static bool plan_regs(const FftPlan &pl)
 }
 
 // x-line pairs per block (register path): threads = p*n/8 <= 1024, LDS <= LDS_CAP.
-static constexpr int rpairs_n(int n)
-{
-    int p = 8;
-    while (p > 1 && p * (n / 8) > FFT_MAX_THREADS) p >>= 1;
-    while (p > 1 && (size_t)p * rline_stride(n) * sizeof(double2) > LDS_CAP) p >>= 1;
-    while (p > 1 && (n * n / 2) % p != 0) p >>= 1;
-    return p;
-}
 static int rpairs(const FftPlan &pl) { return rpairs_n(pl.n); }
 
 // kx columns per block (register path).
-static constexpr int rtile_n(int n)
-{
-    int t = 8;
-    while (t > 1 && t * (n / 8) > FFT_MAX_THREADS) t >>= 1;
-    while (t > 1 && (size_t)t * n * sizeof(double2) > LDS_CAP) t >>= 1;
-    return t;
-}
 static int rtile(const FftPlan &pl) { return rtile_n(pl.n); }
 
-// The register kernels are compiled for n = 2^L, 7 <= L <= 10, at their
-// default launch shapes (rtile, rpairs), with the stage plan and the column
-// tile as constants; other n and shapes (a tuned shape, say) run the kernels
-// compiled for any n, which read the plan at run time.  reg_log2 gives L when
-// pl is the compiled plan, else 0.
-constexpr int REG_LMIN = 7, REG_LMAX = 10;
+// The register kernels are compiled for n = 2^L, 6 <= L <= 10, with the stage
+// plan as constants; the column passes also with their default tile (rtile)
+// as a constant, so another tile (a tuned one, say) runs the kernels compiled
+// for any n, which read the plan at run time.  The x passes take any number of
+// line pairs per block.  reg_log2 gives L when pl is the compiled plan, else 0.
+constexpr int REG_LMIN = 6, REG_LMAX = 10;
 static constexpr int rtile_c(int L) { return rtile_n(1 << L); }
-static constexpr int rpairs_c(int L) { return rpairs_n(1 << L); }
 static int reg_log2(const FftPlan &pl)
 {
     int L = 0;
@@ -2455,6 +2458,7 @@ template <typename Fn>
 static void with_log2(int L, Fn &&fn)
 {
     switch (L) {
+    case 6: fn(std::integral_constant<int, 6>()); break;
     case 7: fn(std::integral_constant<int, 7>()); break;
     case 8: fn(std::integral_constant<int, 8>()); break;
     case 9: fn(std::integral_constant<int, 9>()); break;
@@ -2462,16 +2466,21 @@ static void with_log2(int L, Fn &&fn)
     default: fn(std::integral_constant<int, 0>()); break;
     }
 }
-// L for a column pass of tile tkx and for an x pass of ppb line pairs per block.
+// The same for the gang kernels, whose boxes are at most 128^3.
+template <typename Fn>
+static void with_log2_gang(int L, Fn &&fn)
+{
+    switch (L) {
+    case 6: fn(std::integral_constant<int, 6>()); break;
+    case 7: fn(std::integral_constant<int, 7>()); break;
+    default: fn(std::integral_constant<int, 0>()); break;
+    }
+}
+// L for a column pass of tile tkx.
 static int reg_log2_cols(const FftPlan &pl, int tkx)
 {
     const int L = reg_log2(pl);
     return L && tkx == rtile_c(L) ? L : 0;
-}
-static int reg_log2_rows(const FftPlan &pl, int ppb)
-{
-    const int L = reg_log2(pl);
-    return L && ppb == rpairs_c(L) ? L : 0;
 }
 
 static int fft_threads(const FftPlan &pl, int nlines)
@@ -2525,7 +2534,7 @@ void launch_xfwd(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
         const int ppb = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
         const dim3 grid(pairs / ppb), blk(ppb * (c.n / 8));
-        with_log2(reg_log2_rows(pl, ppb), [&](auto LC) {
+        with_log2(reg_log2(pl), [&](auto LC) {
             constexpr int L = decltype(LC)::value;
             if (c.mixed)
                 hipLaunchKernelGGL((k_xfwd_r<float, L>), grid, blk, shm, s, (const float *)f.us,
@@ -2611,7 +2620,7 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
     if (plan_regs(pl)) {
         const int ppb = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl), pairs = c.n * c.n / 2;
         const size_t shm = sizeof(double2) * (size_t)ppb * rline_stride(c.n);
-        with_log2(reg_log2_rows(pl, ppb), [&](auto LC) {
+        with_log2(reg_log2(pl), [&](auto LC) {
             hipLaunchKernelGGL((k_xinv_r<decltype(LC)::value>), dim3(pairs / ppb), dim3(ppb * (c.n / 8)), shm, s,
                                f.spec, f.p, f.tw, f.sc, pl, ppb, f.ld);
         });
@@ -2690,19 +2699,33 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
         const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl);
         const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl);
         const size_t shx = sizeof(double2) * (size_t)xp * rline_stride(n);
-        if (c.mixed)
-            hipLaunchKernelGGL((kg_xfwd<float, true>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g, pl, xp,
-                               ld);
-        else
-            hipLaunchKernelGGL((kg_xfwd<double, true>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g, pl, xp,
-                               ld);
+        with_log2_gang(reg_log2(pl), [&](auto LC) {
+            constexpr int L = decltype(LC)::value;
+            if (c.mixed)
+                hipLaunchKernelGGL((kg_xfwd<float, true, L>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g,
+                                   pl, xp, ld);
+            else
+                hipLaunchKernelGGL((kg_xfwd<double, true, L>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g,
+                                   pl, xp, ld);
+        });
         const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
         const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
-        hipLaunchKernelGGL((kg_ypass<-1, true>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL(kg_zpoisson<true>, gz, dim3(tz * (n / 8)), shz, s, g, pl, ld, tz);
-        hipLaunchKernelGGL((kg_ypass<1, true>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL(kg_xinv<true>, dim3(pairs / xip, nbox), dim3(xip * (n / 8)),
-                           sizeof(double2) * (size_t)xip * rline_stride(n), s, g, pl, xip, ld);
+        with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
+            hipLaunchKernelGGL((kg_ypass<-1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld,
+                               ty);
+        });
+        with_log2_gang(reg_log2_cols(pl, tz), [&](auto LC) {
+            hipLaunchKernelGGL((kg_zpoisson<true, decltype(LC)::value>), gz, dim3(tz * (n / 8)), shz, s, g, pl, ld,
+                               tz);
+        });
+        with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
+            hipLaunchKernelGGL((kg_ypass<1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld,
+                               ty);
+        });
+        with_log2_gang(reg_log2(pl), [&](auto LC) {
+            hipLaunchKernelGGL((kg_xinv<true, decltype(LC)::value>), dim3(pairs / xip, nbox), dim3(xip * (n / 8)),
+                               sizeof(double2) * (size_t)xip * rline_stride(n), s, g, pl, xip, ld);
+        });
     } else {
         const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : pairs_per_block(pl);
         const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : pairs_per_block(pl);
@@ -2710,17 +2733,17 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &
         const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
         const size_t shx = sizeof(double2) * (size_t)xp * (n + 1);
         if (c.mixed)
-            hipLaunchKernelGGL((kg_xfwd<float, false>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
+            hipLaunchKernelGGL((kg_xfwd<float, false, 0>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
                                pl, xp, ld);
         else
-            hipLaunchKernelGGL((kg_xfwd<double, false>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
+            hipLaunchKernelGGL((kg_xfwd<double, false, 0>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
                                pl, xp, ld);
         const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
         const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
-        hipLaunchKernelGGL((kg_ypass<-1, false>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL(kg_zpoisson<false>, gz, dim3(fft_threads(pl, tz)), shz, s, g, pl, ld, tz);
-        hipLaunchKernelGGL((kg_ypass<1, false>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL(kg_xinv<false>, dim3(pairs / xip, nbox), dim3(fft_threads(pl, xip)),
+        hipLaunchKernelGGL((kg_ypass<-1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL((kg_zpoisson<false, 0>), gz, dim3(fft_threads(pl, tz)), shz, s, g, pl, ld, tz);
+        hipLaunchKernelGGL((kg_ypass<1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
+        hipLaunchKernelGGL((kg_xinv<false, 0>), dim3(pairs / xip, nbox), dim3(fft_threads(pl, xip)),
                            sizeof(double2) * (size_t)xip * (n + 1), s, g, pl, xip, ld);
     }
     const dim3 gp((unsigned)nblocks((size_t)n * n * 64, BLK), nbox);

@@ -1,12 +1,14 @@
-"""The FFT passes compiled for n = 2^L (128 <= n <= 1024) at their default
-launch shapes against the kernels compiled for any n.
+"""The FFT passes compiled for n = 2^L (64 <= n <= 1024) against the kernels
+compiled for any n.
 
-The compiled kernels take the stage plan (radices 8,4,4 at 128; 8,8,4 at 256;
-8,8,8 at 512; 8,8,4,4 at 1024) and the column tile as constants; a non-default
-launch shape (one x-line pair per block, 2-column y/z tiles) runs the kernels
-that read the plan at run time.  Same inputs, same arithmetic: the fields must
-agree bit for bit.  128^3 (the smallest compiled size, plan 8,4,4) is also
-checked against the oracle; 256^3 and 512^3 are in the configuration tests.
+The compiled kernels take the stage plan (radices 8,8 at 64; 8,4,4 at 128;
+8,8,4 at 256; 8,8,8 at 512; 8,8,4,4 at 1024) as constants, the y/z passes
+also their default column tile.  A 2-column y/z tile runs the kernels that
+read the plan at run time; the x passes stay compiled for any number of line
+pairs per block, and run here with one pair instead of the default.  Same
+inputs, same arithmetic: the fields must agree bit for bit.  128^3 (plan
+8,4,4) is also checked against the oracle; 64^3, 256^3 and 512^3 are in the
+parity and configuration tests.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,7 @@ from oracle import orc
 
 pytestmark = pytest.mark.gpu
 
-GENERIC = dict(x_pairs=1, xinv_pairs=1, y_tile=2, z_tile=2)
+OTHER_SHAPES = dict(x_pairs=1, xinv_pairs=1, y_tile=2, z_tile=2)
 
 
 def _drops_case(n, seed):
@@ -49,7 +51,7 @@ def test_128_compiled_plan_bit_identical_to_oracle():
 def test_128_compiled_equals_generic(precision):
     kw, fields = _drops_case(128, 41)
     out = []
-    for shapes in ({}, GENERIC):
+    for shapes in ({}, OTHER_SHAPES):
         with Solver(**kw, precision=precision) as s:
             if shapes:
                 s.set_tuning(**shapes)
@@ -68,7 +70,7 @@ def test_1024_compiled_equals_generic():
     dx = 2 * np.pi / (n - 1)
     kw = dict(nx=n, dt=0.06 * dx, mu=0.001, f1=1.0, f2=1.0, f3=1.0, k0=2.0, phiflag=0, inflow=1)
     res = []
-    for shapes in ({}, GENERIC):
+    for shapes in ({}, OTHER_SHAPES):
         print(f"1024^3 shapes={shapes or 'default'}", flush=True)
         with Solver(**kw) as s:
             s.init_random_velocity(seed=99, k_peak=4.0, urms=1.0)
