@@ -197,20 +197,24 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     T z_m1, z_0;
     T hf_p1 = 0, hf_p2 = 0, hfpre = 0;
     T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the output plane
+    T nwp = 0;                                         // w of the plane above it
     // the x+1 and y+1 neighbours of the own cell, as offsets from it (periodic)
     const unsigned xpo = (unsigned)wrap(i + 1, n) + (unsigned)n * (unsigned)wrap(j, n);
     const unsigned ypo = (unsigned)wrap(i, n) + (unsigned)n * (unsigned)wrap(j + 1, n);
-    auto fetch_uvw = [&](int m) {
+    // u, v of plane m and w of plane m+1 (w of plane m was the previous
+    // call's w of m+1: first = true loads both)
+    auto fetch_uvw = [&](int m, bool first) {
         if (active) {
-            // the plane offset is opaque: otherwise the compiler hoists
-            // base + own out of the march as five 64-bit vector pointers,
-            // which spill
-            size_t z = nn * (size_t)m;
-            asm volatile("" : "+s"(z));
-            const T *up = u + z, *vp = v + z, *wp = w + z;
+            // the plane offsets are opaque: otherwise the compiler hoists
+            // base + own out of the march as 64-bit vector pointers, which
+            // spill
+            size_t z = nn * (size_t)m, zp = nn * (size_t)(m + 1 == n ? 0 : m + 1);
+            asm volatile("" : "+s"(z), "+s"(zp));
+            const T *up = u + z, *vp = v + z;
             nu0 = ld_off(up, own); nuxp = ld_off(up, xpo);
             nv0 = ld_off(vp, own); nvyp = ld_off(vp, ypo);
-            nw0 = ld_off(wp, own);
+            nw0 = first ? ld_off(w + z, own) : nwp;
+            nwp = ld_off(w + zp, own);
         }
     };
 
@@ -224,7 +228,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             hfm1 = ld_phi(hoff, k0 - 1); hf0 = ld_phi(hoff, k0); hf_p1 = ld_phi(hoff, k0 + 1);
             hf_p2 = ld_phi(hoff, k0 + 2);
         }
-        fetch_uvw(k0);
+        fetch_uvw(k0, true);
         const T qm2 = psi_of(fm2), qm1 = psi_of(fm1), q0 = psi_of(f0), qp1 = psi_of(f_p1);
         T hqm1 = 0, hq0 = 0, hqp1 = 0;
         if (hk) { hqm1 = psi_of(hfm1); hq0 = psi_of(hf0); hqp1 = psi_of(hf_p1); }
@@ -279,8 +283,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             T fxp = phL[s0][sI + 1], fxm = phL[s0][sI - 1];
             T fyp = phL[s0][sI + ASX], fym = phL[s0][sI - ASX];
             T fzp = f_p1, fzm = phL[s1][sI];
-            T wkp_raw;
-            wkp_raw = ld_off(w + nn * (size_t)(k + 1 == n ? 0 : k + 1), own);
+            const T wkp_raw = nwp;
             T uip = nuxp - um, u0 = nu0 - um;
             T vjp = nvyp - vm, v0 = nv0 - vm;
             T wkp = wkp_raw - wm, w0 = nw0 - wm;
@@ -314,7 +317,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 __builtin_nontemporal_store(f0 + (T)c.dt * r2, at_off(phin + nn * (size_t)k, own));
             }
         }
-        if (k + 1 < kend) fetch_uvw(k + 1);
+        if (k + 1 < kend) fetch_uvw(k + 1, false);
         // psi(k+2) and phi(k+1) for the x/y neighbours of the next iteration
         psiL[s0][pI] = q_p2;
         phL[s1][sI] = f_p1;
@@ -327,9 +330,9 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     }
 }
 
-// FP32: 8 waves per SIMD (64 VGPRs).  FP64: 126 VGPRs, 4 waves per SIMD (two
-// blocks per CU); held to the 80 VGPRs of 6 waves (three blocks, which the
-// 44 KB of LDS would allow) the pass spills and runs 2.3x slower.
+// FP32: 8 waves per SIMD (62 VGPRs).  FP64: 102 VGPRs, two blocks per CU;
+// held to the 80 VGPRs of 6 waves (three blocks, which the 44 KB of LDS would
+// allow) the pass spills and runs 2.3x slower.
 template <typename T>
 __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
@@ -1578,7 +1581,9 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__res
                                                             const Scalars *__restrict__ sc, FftPlan pl, int ppb,
                                                             int ld)
 {
-    k_xinv_r_body<L>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
+    // last pairs first: the y inverse pass wrote the highest planes last, and
+    // some of them are still in the memory-side cache
+    k_xinv_r_body<L>(make_uint3(gridDim.x - 1 - blockIdx.x, blockIdx.y, blockIdx.z), spec, p, tw, sc, pl, ppb, ld);
 }
 
 
