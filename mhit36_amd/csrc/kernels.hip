@@ -55,8 +55,9 @@ __device__ __forceinline__ uint3 xcd_block()
 // Iteration k: psi(k+2) from phi loaded one iteration earlier; s(k+1); the
 // output plane k: convective and diffusive terms (A4, A5), the sharpening
 // divergence and phi^{n+1} = phi + dt*rhs (A10) written to a second buffer;
-// then psi(k+2) and phi(k+1) into LDS, and one barrier.  u, v, w of plane k+1
-// and phi of plane k+3 are loaded while plane k is computed.  HBM traffic:
+// then psi(k+2) and phi(k+1) into LDS, and one barrier.  u, v of plane k+1, w
+// of plane k+2 and phi of plane k+3 are loaded at the start of iteration k, a
+// whole iteration ahead of their use.  HBM traffic:
 // u, v, w, phi read once (plus halos), phi^{n+1} written once; psi and s
 // never leave the chip.
 // ---------------------------------------------------------------------------
@@ -198,12 +199,12 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     T hf_p1 = 0, hf_p2 = 0, hfpre = 0;
     T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the output plane
     T nwp = 0;                                         // w of the plane above it
+    T pu0 = 0, puxp = 0, pv0 = 0, pvyp = 0, pwp = 0;   // the same one plane up, in flight
     // the x+1 and y+1 neighbours of the own cell, as offsets from it (periodic)
     const unsigned xpo = (unsigned)wrap(i + 1, n) + (unsigned)n * (unsigned)wrap(j, n);
     const unsigned ypo = (unsigned)wrap(i, n) + (unsigned)n * (unsigned)wrap(j + 1, n);
-    // u, v of plane m and w of plane m+1 (w of plane m was the previous
-    // call's w of m+1: first = true loads both)
-    auto fetch_uvw = [&](int m, bool first) {
+    // u, v of plane m and w of plane m+1 into the in-flight registers
+    auto fetch_uvw = [&](int m) {
         if (active) {
             // the plane offsets are opaque: otherwise the compiler hoists
             // base + own out of the march as 64-bit vector pointers, which
@@ -211,10 +212,9 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             size_t z = nn * (size_t)m, zp = nn * (size_t)(m + 1 == n ? 0 : m + 1);
             asm volatile("" : "+s"(z), "+s"(zp));
             const T *up = u + z, *vp = v + z;
-            nu0 = ld_off(up, own); nuxp = ld_off(up, xpo);
-            nv0 = ld_off(vp, own); nvyp = ld_off(vp, ypo);
-            nw0 = first ? ld_off(w + z, own) : nwp;
-            nwp = ld_off(w + zp, own);
+            pu0 = ld_off(up, own); puxp = ld_off(up, xpo);
+            pv0 = ld_off(vp, own); pvyp = ld_off(vp, ypo);
+            pwp = ld_off(w + zp, own);
         }
     };
 
@@ -228,7 +228,9 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
             hfm1 = ld_phi(hoff, k0 - 1); hf0 = ld_phi(hoff, k0); hf_p1 = ld_phi(hoff, k0 + 1);
             hf_p2 = ld_phi(hoff, k0 + 2);
         }
-        fetch_uvw(k0, true);
+        if (active) nwp = ld_off(w + nn * (size_t)k0, own);
+        fetch_uvw(k0);
+        nu0 = pu0; nuxp = puxp; nv0 = pv0; nvyp = pvyp; nw0 = nwp; nwp = pwp;
         const T qm2 = psi_of(fm2), qm1 = psi_of(fm1), q0 = psi_of(f0), qp1 = psi_of(f_p1);
         T hqm1 = 0, hq0 = 0, hqp1 = 0;
         if (hk) { hqm1 = psi_of(hfm1); hq0 = psi_of(hf0); hqp1 = psi_of(hf_p1); }
@@ -265,6 +267,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
         if (hk) hq_p2 = psi_of(hf_p2);
         fpre = ld_phi(own, k + 3);
         if (hk) hfpre = ld_phi(hoff, k + 3);
+        if (k + 1 < kend) fetch_uvw(k + 1);       // used by the next iteration's output
         // s(k+1): x/y parts of the normal from psi(k+1) in LDS, z part from the
         // column (psi(k) from the own slot, psi(k+2) just formed)
         T sx, sy, z_p1;
@@ -317,7 +320,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 __builtin_nontemporal_store(f0 + (T)c.dt * r2, at_off(phin + nn * (size_t)k, own));
             }
         }
-        if (k + 1 < kend) fetch_uvw(k + 1, false);
+        nu0 = pu0; nuxp = puxp; nv0 = pv0; nvyp = pvyp; nw0 = nwp; nwp = pwp;
         // psi(k+2) and phi(k+1) for the x/y neighbours of the next iteration
         psiL[s0][pI] = q_p2;
         phL[s1][sI] = f_p1;
