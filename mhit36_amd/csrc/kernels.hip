@@ -1507,7 +1507,10 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restr
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                                 int tkx)
 {
-    k_zpoisson_r_body<false, L, TK>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld, tkx);
+    // Tiles narrower than a 128-byte line (4 columns at 1024 rows): XCD-first
+    // block order, so the two tiles that share each line run in one L2.
+    const uint3 b = TK != 0 && TK < 8 ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    k_zpoisson_r_body<false, L, TK>(b, spec, tw, ktab, sc, pl, ld, tkx);
 }
 
 
