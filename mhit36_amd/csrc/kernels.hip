@@ -880,7 +880,8 @@ __global__ void __launch_bounds__(PTHREADS) k_momentum_x2(
 // wavefront are contiguous for column tiles.
 // ---------------------------------------------------------------------------
 constexpr int FFT_MAX_THREADS = 1024;
-constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block of the heuristic shapes (2 blocks per CU)
+constexpr size_t LDS_CAP = 65536;   // dynamic LDS bytes per block of the heuristic shapes (2 blocks per CU;
+                                    // register-path column tiles up to twice that, 1 block per CU)
 constexpr size_t LDS_MAX = 163840;  // what a gfx950 workgroup may allocate (tunable shapes)
 
 template <int R, int SG>
@@ -1282,13 +1283,16 @@ __device__ __forceinline__ void fft_lines(double2 (&v)[8], int tj, int line, dou
 __host__ __device__ constexpr int rline_stride(int n) { return n + n / 8 + 1; }
 
 // Default launch shapes of the register kernels: columns per block of the y/z
-// passes and x-line pairs per block (threads = shape * n/8 <= 1024, LDS within
-// LDS_CAP, pairs dividing n*n/2).
+// passes and x-line pairs per block (threads = shape * n/8 <= 1024, pairs
+// dividing n*n/2).  A column tile keeps whole 128-byte lines (8 columns) up to
+// 2*LDS_CAP of LDS, one block per CU: at 1024 rows that beats two blocks of
+// 4-column tiles, whose 64-byte row pieces half-use each line (z pass 5.9 ->
+// 5.4 ms at 1024^3).  Line pairs stay within LDS_CAP.
 static constexpr int rtile_n(int n)
 {
     int t = 8;
     while (t > 1 && t * (n / 8) > FFT_MAX_THREADS) t >>= 1;
-    while (t > 1 && (size_t)t * n * sizeof(double2) > LDS_CAP) t >>= 1;
+    while (t > 1 && (size_t)t * n * sizeof(double2) > 2 * LDS_CAP) t >>= 1;
     return t;
 }
 static constexpr int rpairs_n(int n)
@@ -1507,10 +1511,8 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson_r(double2 *__restr
                                                                 const Scalars *__restrict__ sc, FftPlan pl, int ld,
                                                                 int tkx)
 {
-    // Tiles narrower than a 128-byte line (4 columns at 1024 rows): XCD-first
-    // block order, so the two tiles that share each line run in one L2.
-    const uint3 b = TK != 0 && TK < 8 ? xcd_block() : make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
-    k_zpoisson_r_body<false, L, TK>(b, spec, tw, ktab, sc, pl, ld, tkx);
+    k_zpoisson_r_body<false, L, TK>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), spec, tw, ktab, sc, pl, ld,
+                                    tkx);
 }
 
 
