@@ -333,7 +333,7 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     }
 }
 
-// FP32: 8 waves per SIMD (62 VGPRs).  FP64: 102 VGPRs, two blocks per CU;
+// FP32: 8 waves per SIMD (63 VGPRs).  FP64: 112 VGPRs, two blocks per CU;
 // held to the 80 VGPRs of 6 waves (three blocks, which the 44 KB of LDS would
 // allow) the pass spills and runs 2.3x slower.
 template <typename T>
