@@ -1225,15 +1225,15 @@ __device__ void fft_regs(double2 (&v)[8], int tj, int line, double2 *lds, int ls
     }
 }
 
-// The stage plan of a power-of-two length n = 2^L at compile time: the
-// radices, sub-lengths Ns and twiddle offsets that the host plan builder
+// The stage plan of a power-of-two length n = 2^L, L >= 4, at compile time:
+// the radices, sub-lengths Ns and twiddle offsets that the host plan builder
 // gives for that n (radix-8 stages, then two 4s when L mod 3 = 1, one 4 when
 // L mod 3 = 2).  With L known the stage loop unrolls and every index of the
 // exchanges (jj / Ns, jj % Ns, the line strides) is a shift or a mask; the
 // generic fft_regs computes them from the runtime plan with integer divides.
 __host__ __device__ constexpr int cplan_nst(int L)
 {
-    return L % 3 == 0 ? L / 3 : L % 3 == 2 ? L / 3 + 1 : L / 3 + 1;
+    return L % 3 == 0 ? L / 3 : L / 3 + 1;
 }
 __host__ __device__ constexpr int cplan_R(int L, int s)
 {
@@ -1256,6 +1256,7 @@ template <int SG, bool XL, int L, int S = 0>
 __device__ __forceinline__ void fft_regs_c(double2 (&v)[8], int tj, int line, double2 *lds, int lstride,
                                            int pstride, const double2 *__restrict__ tw)
 {
+    static_assert(L >= 4, "the compile-time plan covers n >= 16");
     constexpr int LM = (1 << L) / 8, R = cplan_R(L, S), Ns = cplan_Ns(L, S);
     if constexpr (S > 0) {
 #pragma unroll
