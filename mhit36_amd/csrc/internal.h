@@ -18,6 +18,25 @@ struct FftPlan {
     int twoff[MAXSTAGES];
 };
 
+// The radices of the plan for n = 2^a 3^b (radix-8 stages, then 4s, then a 2,
+// then 3s) into rad; returns the stage count, 0 for any other n.
+constexpr int plan_radices(int n, int *rad)
+{
+    int a = 0, b = 0, m = n;
+    if (n < 2) return 0;
+    while (m % 2 == 0) { m /= 2; a++; }
+    while (m % 3 == 0) { m /= 3; b++; }
+    if (m != 1 || a < 1) return 0;
+    int ns = 0, q = a / 3, r = a % 3;
+    if (r == 1 && q >= 1) { q -= 1; r = 4; }
+    for (int i = 0; i < q; i++) rad[ns++] = 8;
+    if (r == 2) rad[ns++] = 4;
+    if (r == 4) { rad[ns++] = 4; rad[ns++] = 4; }
+    if (r == 1) rad[ns++] = 2;
+    for (int i = 0; i < b; i++) rad[ns++] = 3;
+    return ns;
+}
+
 // Device-resident scalars carried from one step to the next.  u, v and w are
 // stored before their mean is removed; every reader subtracts umean/vmean/wmean
 // (the same rounding as storing u - umean).

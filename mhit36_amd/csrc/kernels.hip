@@ -1252,6 +1252,23 @@ __host__ __device__ constexpr int cplan_twoff(int L, int s)
     return o;
 }
 
+// The compile-time plan is the host's plan (plan_radices) for every compiled
+// length; Ns and the twiddle offsets follow from the radices the same way in
+// both.
+constexpr bool cplan_is_host_plan(int L)
+{
+    int rad[MAXSTAGES] = {};
+    const int ns = plan_radices(1 << L, rad);
+    if (ns != cplan_nst(L)) return false;
+    for (int st = 0; st < ns; st++)
+        if (rad[st] != cplan_R(L, st)) return false;
+    return true;
+}
+static_assert(cplan_is_host_plan(4) && cplan_is_host_plan(5) && cplan_is_host_plan(6) && cplan_is_host_plan(7) &&
+                  cplan_is_host_plan(8) && cplan_is_host_plan(9) && cplan_is_host_plan(10) &&
+                  cplan_is_host_plan(11),
+              "compile-time FFT plan differs from the host plan");
+
 template <int SG, bool XL, int L, int S = 0>
 __device__ __forceinline__ void fft_regs_c(double2 (&v)[8], int tj, int line, double2 *lds, int lstride,
                                            int pstride, const double2 *__restrict__ tw)
