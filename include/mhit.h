@@ -138,12 +138,14 @@ int mhit_get_field(mhit_ctx *ctx, int fieldn, double *host);
 int mhit_step(mhit_ctx *ctx, int nsteps, mhit_diag *diag);
 
 /* Step several boxes together (no counterpart in the reference, which runs one
- * box per job): ctx[0..nbox) are distinct contexts of one nx, precision and
- * phiflag (MHIT_ERR_ARG otherwise), at most MHIT_BATCH_MAX of them.  Each pass
- * of a step is one launch for all boxes, so small boxes fill the GPU; every
- * box's fields are bit-identical to mhit_step on that box alone.  Boxes may
- * differ in every other parameter and in their step count (an Euler first
- * step beside AB2 steps).  diag[b] (may be NULL) is box b's mhit_step result;
+ * box per job): ctx[0..nbox) are distinct contexts of one precision and
+ * phiflag on one device (MHIT_ERR_ARG otherwise), at most MHIT_BATCH_MAX of
+ * them, of one or several sizes nx.  The phase-field, momentum, projection and
+ * reduction passes of a step are one launch for all boxes, the FFT passes one
+ * launch per size, so small boxes fill the GPU; every box's fields are
+ * bit-identical to mhit_step on that box alone.  Boxes may differ in every
+ * other parameter and in their step count (an Euler first step beside AB2
+ * steps).  diag[b] (may be NULL) is box b's mhit_step result;
  * a box that stops (CFL, NaN) or was already stopped does no further steps
  * while the others go on.  Returns MHIT_OK, or the first box's error.  The
  * launches go on ctx[0]'s stream; returns after the device work completes. */

@@ -108,13 +108,21 @@ __host__ __device__ inline size_t spec_row(int k, int j, int n, int ld)
     return (size_t)k * spec_plane(n, ld) + (size_t)j * ld;
 }
 
-// One box of a gang: the boxes of one n, precision and phiflag that a single
-// launch per pass steps together.  An array of these sits in device memory;
-// phi/phin are as the step being launched reads and writes them.
+// One box of a gang: boxes of one precision and phiflag, of one or several
+// sizes n, stepped together.  An array of these, sorted by n, sits in device
+// memory; phi/phin are as the step being launched reads and writes them.
 constexpr int GANG_MAX = 16;
 struct GangBox {
     Fields f;
     Consts c;
+};
+// The phase-field, momentum, projection and plane-reduction passes take one
+// launch for the whole gang, whatever the sizes: box b's blocks are
+// first[b] .. first[b+1]-1 of the launch's 1-D grid.  Passed by value.
+enum GangPass { GP_ACDI, GP_MOM, GP_PROJECT, GP_PLANES, GP_COUNT };
+struct GangGrid {
+    int first[GANG_MAX + 1];   // first[nbox]: the launch's block count
+    int kz[GANG_MAX];          // z planes per block (the marching passes)
 };
 
 // Launch helpers (kernels.hip).  Each returns the name and the algorithmic
@@ -148,10 +156,26 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_reduce(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
 void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li);
-// One step of nbox boxes (descriptors g in device memory, each of c's n,
-// precision and phiflag; bit b of euler: box b takes its Euler first step).
-void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &c, const FftPlan &pl, int ld,
-                      const Tuning &tu, hipStream_t s);
+// The launches of a gang step: the grid of each shared pass and, per size n,
+// its boxes [off, off + cnt) with the FFT plan, spectrum row length and
+// launch shapes of that size (the FFT passes take one launch per size).
+struct GangSize {
+    int off, cnt, n, ld;
+    FftPlan plan;
+    Tuning tune;
+};
+struct GangLaunch {
+    int mixed, phiflag;
+    GangGrid grid[GP_COUNT];
+    int nsize;
+    GangSize size[GANG_MAX];
+};
+// The launch plan gl of the nbox descriptors d (sorted by n, one precision and
+// phiflag); sizes[q] gives the plan, ld and shapes of descriptor q's size.
+void gang_layout(const GangBox *d, int nbox, const GangSize *sizes, GangLaunch *gl);
+// One step of the gang (descriptors g in device memory, laid out by
+// gang_layout; bit b of euler: box b takes its Euler first step).
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s);
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s);
 // Drop counting: label the phi >= 0.5 cells (26-connected, periodic) by the
 // smallest linear index of their set; list the roots; count cells per root

@@ -1797,21 +1797,38 @@ __global__ void __launch_bounds__(64) k_finalize(const double *__restrict__ psum
 // buffers and constants from the descriptor array g and runs the one-box code
 // unchanged, so a box gives the same bits in a gang as alone.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned gang_fold(uint3 &b, int nbox)
+// The place of this block in a gang's shared launch (a 1-D grid): its box,
+// the last one whose first block is at or below the block's index, and its
+// index among that box's blocks.  Indices run XCD-first (as in xcd_block), so
+// each XCD works through consecutive blocks, mostly of one box.
+__device__ __forceinline__ unsigned gang_block(const GangGrid &gg, int nbox, unsigned &local)
 {
-    const unsigned gz = gridDim.z / (unsigned)nbox, box = b.z / gz;
-    b.z -= box * gz;
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+    unsigned box = 0;
+    for (int q = 1; q < nbox; q++)
+        if ((unsigned)gg.first[q] <= b) box = (unsigned)q;
+    local = b - (unsigned)gg.first[box];
     return box;
+}
+
+// Tile coordinates of a box's block `l` in a grid of gx x gy tiles per slab.
+__device__ __forceinline__ uint3 tile_of(unsigned l, unsigned gx, unsigned gy)
+{
+    return make_uint3(l % gx, (l / gx) % gy, l / (gx * gy));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) kg_acdi(const GangBox *__restrict__ g, int nbox,
-                                                                           int kz)
+                                                                           GangGrid gg)
 {
-    uint3 b = xcd_block();
-    const GangBox &B = g[gang_fold(b, nbox)];
-    k_acdi_body<T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phi, (T *)B.f.phin,
-                   B.f.sc, B.f.mtab, B.c, kz);
+    unsigned l;
+    const unsigned box = gang_block(gg, nbox, l);
+    const GangBox &B = g[box];
+    const unsigned n = (unsigned)B.c.n;
+    k_acdi_body<T>(tile_of(l, (n + ATX - 1) / ATX, (n + ATY - 1) / ATY), (const T *)B.f.u, (const T *)B.f.v,
+                   (const T *)B.f.w, (const T *)B.f.phi, (T *)B.f.phin, B.f.sc, B.f.mtab, B.c, gg.kz[box]);
 }
 
 // euler: bit b set when box b takes its first (Euler) step, main.f90:323-324.
@@ -1820,28 +1837,34 @@ __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) kg_acdi(cons
 // At most 128 VGPRs (4 waves per SIMD), as the one-box kernel compiles to.
 template <int PHI, typename T>
 __global__ void __launch_bounds__(MTHREADS, 4) kg_momentum(const GangBox *__restrict__ g, int nbox, unsigned euler,
-                                                        int kz)
+                                                        GangGrid gg)
 {
-    uint3 b = xcd_block();
-    const unsigned box = gang_fold(b, nbox);
+    unsigned l;
+    const unsigned box = gang_block(gg, nbox, l);
     const GangBox &B = g[box];
     const bool eu = (euler >> box) & 1u;
-    k_momentum_body<PHI, T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin,
-                            (T *)B.f.us, (T *)B.f.vs, (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo,
-                            B.f.sinf, B.f.cosf, B.f.sc, B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
+    const unsigned n = (unsigned)B.c.n;
+    const int kz = gg.kz[box];
+    k_momentum_body<PHI, T>(tile_of(l, (n + MTX - 1) / MTX, (n + MTY - 1) / MTY), (const T *)B.f.u,
+                            (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin, (T *)B.f.us, (T *)B.f.vs,
+                            (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo, B.f.sinf, B.f.cosf, B.f.sc,
+                            B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
 }
 
 template <int PHI, typename T>
 __global__ void __launch_bounds__(PTHREADS) kg_momentum_x2(const GangBox *__restrict__ g, int nbox, unsigned euler,
-                                                           int kz)
+                                                           GangGrid gg)
 {
-    uint3 b = xcd_block();
-    const unsigned box = gang_fold(b, nbox);
+    unsigned l;
+    const unsigned box = gang_block(gg, nbox, l);
     const GangBox &B = g[box];
     const bool eu = (euler >> box) & 1u;
-    k_momentum_x2_body<PHI, T>(b, (const T *)B.f.u, (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin,
-                               (T *)B.f.us, (T *)B.f.vs, (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo,
-                               B.f.sinf, B.f.cosf, B.f.sc, B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
+    const unsigned n = (unsigned)B.c.n;
+    const int kz = gg.kz[box];
+    k_momentum_x2_body<PHI, T>(tile_of(l, (n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY), (const T *)B.f.u,
+                               (const T *)B.f.v, (const T *)B.f.w, (const T *)B.f.phin, (T *)B.f.us, (T *)B.f.vs,
+                               (T *)B.f.ws, (T *)B.f.ruo, (T *)B.f.rvo, (T *)B.f.rwo, B.f.sinf, B.f.cosf, B.f.sc,
+                               B.c, eu ? 1.0 : 1.5, eu ? 0.0 : 0.5, kz);
 }
 
 // FFT passes: box = the first grid dimension the one-box launch leaves unused.
@@ -1893,17 +1916,20 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) kg_xinv(const GangBox *__rest
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BLK) kg_project(const GangBox *__restrict__ g)
+__global__ void __launch_bounds__(BLK) kg_project(const GangBox *__restrict__ g, int nbox, GangGrid gg)
 {
-    const GangBox &B = g[blockIdx.y];
-    k_project_body<T>(make_uint3(blockIdx.x, 0, 0), (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.p,
+    unsigned l;
+    const GangBox &B = g[gang_block(gg, nbox, l)];
+    k_project_body<T>(make_uint3(l, 0, 0), (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.p,
                       (T *)B.f.u, (T *)B.f.v, (T *)B.f.w, B.f.lsum, B.f.lmax, B.f.sc, B.c);
 }
 
-__global__ void __launch_bounds__(64) kg_reduce_planes(const GangBox *__restrict__ g)
+__global__ void __launch_bounds__(64) kg_reduce_planes(const GangBox *__restrict__ g, int nbox, GangGrid gg)
 {
-    const GangBox &B = g[blockIdx.z];
-    k_reduce_planes_body(make_uint3(blockIdx.x, blockIdx.y, 0), B.f.lsum, B.f.lmax, B.f.psum, B.f.pmax, B.f.sc,
+    unsigned l;
+    const GangBox &B = g[gang_block(gg, nbox, l)];
+    const unsigned n = (unsigned)B.c.n;
+    k_reduce_planes_body(make_uint3(l % n, l / n, 0), B.f.lsum, B.f.lmax, B.f.psum, B.f.pmax, B.f.sc,
                          B.c.n);
 }
 
@@ -2691,98 +2717,132 @@ void launch_finalize(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo
     li->name = "finalize"; li->bytes = 6.0 * 8.0 * c.n;
 }
 
-// One step of a gang (device descriptors g[0..nbox), all of c's n, precision
-// and phiflag), with the same launch shapes as the one-box passes above.
-void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const Consts &c, const FftPlan &pl, int ld,
-                      const Tuning &tu, hipStream_t s)
+// Block layout of a gang step: each shared pass (phase field, momentum,
+// projection, plane sums) is one 1-D launch over the blocks of every box, a
+// box's blocks consecutive; the boxes' z slabs are sized from the gang's box
+// count (more boxes, thicker slabs and fewer recomputed halo planes).  Boxes
+// of one size form one entry of gl->size for the FFT passes.
+void gang_layout(const GangBox *d, int nbox, const GangSize *sizes, GangLaunch *gl)
 {
-    const int n = c.n, nh = n / 2 + 1, pairs = n * n / 2;
-    // z planes per block from the gang's total block count (the boxes give the
-    // parallelism that thin slabs give one box); the FFT shapes are per line
-    // or column and keep the box's tuned values.
-    if (c.phiflag) {
-        const int kz = march_kz(n, ATX, ATY, nbox, 1024);
-        const dim3 grid((n + ATX - 1) / ATX, (n + ATY - 1) / ATY, (n / kz) * nbox);
-        if (c.mixed)
-            hipLaunchKernelGGL(kg_acdi<float>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
+    *gl = GangLaunch{};
+    gl->mixed = d[0].c.mixed;
+    gl->phiflag = d[0].c.phiflag;
+    for (int b = 0; b < nbox; b++) {
+        const int n = d[b].c.n;
+        const int ka = march_kz(n, ATX, ATY, nbox, 1024), km = march_kz(n, MTX, MTY, nbox, 1024);
+        gl->grid[GP_ACDI].kz[b] = ka;
+        gl->grid[GP_MOM].kz[b] = km;
+        const int blocks[GP_COUNT] = {
+            ((n + ATX - 1) / ATX) * ((n + ATY - 1) / ATY) * (n / ka),
+            gl->mixed ? ((n + 2 * PTX - 1) / (2 * PTX)) * ((n + PTY - 1) / PTY) * (n / km)
+                      : ((n + MTX - 1) / MTX) * ((n + MTY - 1) / MTY) * (n / km),
+            (int)nblocks((size_t)n * n * 64, BLK), 3 * n};
+        for (int q = 0; q < GP_COUNT; q++) gl->grid[q].first[b + 1] = gl->grid[q].first[b] + blocks[q];
+        if (gl->nsize == 0 || gl->size[gl->nsize - 1].n != n) {
+            GangSize &z = gl->size[gl->nsize++];
+            z = sizes[b];
+            z.off = b;
+            z.cnt = 0;
+            z.n = n;
+        }
+        gl->size[gl->nsize - 1].cnt++;
+    }
+}
+
+// One step of a gang laid out by gang_layout (device descriptors g[0..nbox)),
+// with the launch shapes of the one-box passes above: the shared passes once,
+// the FFT passes once per size, on that size's boxes.
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s)
+{
+    if (gl.phiflag) {
+        const GangGrid &gg = gl.grid[GP_ACDI];
+        const dim3 grid((unsigned)gg.first[nbox]);
+        if (gl.mixed)
+            hipLaunchKernelGGL(kg_acdi<float>, grid, dim3(ATHREADS), 0, s, g, nbox, gg);
         else
-            hipLaunchKernelGGL(kg_acdi<double>, grid, dim3(ATHREADS), 0, s, g, nbox, kz);
+            hipLaunchKernelGGL(kg_acdi<double>, grid, dim3(ATHREADS), 0, s, g, nbox, gg);
     }
     {
-        const int kz = march_kz(n, MTX, MTY, nbox, 1024);
-        if (c.mixed) {
-            const dim3 g2((n + 2 * PTX - 1) / (2 * PTX), (n + PTY - 1) / PTY, (n / kz) * nbox);
-            if (c.phiflag)
-                hipLaunchKernelGGL((kg_momentum_x2<1, float>), g2, dim3(PTHREADS), 0, s, g, nbox, euler, kz);
+        const GangGrid &gg = gl.grid[GP_MOM];
+        const dim3 grid((unsigned)gg.first[nbox]);
+        if (gl.mixed) {
+            if (gl.phiflag)
+                hipLaunchKernelGGL((kg_momentum_x2<1, float>), grid, dim3(PTHREADS), 0, s, g, nbox, euler, gg);
             else
-                hipLaunchKernelGGL((kg_momentum_x2<0, float>), g2, dim3(PTHREADS), 0, s, g, nbox, euler, kz);
+                hipLaunchKernelGGL((kg_momentum_x2<0, float>), grid, dim3(PTHREADS), 0, s, g, nbox, euler, gg);
         } else {
-            const dim3 grid((n + MTX - 1) / MTX, (n + MTY - 1) / MTY, (n / kz) * nbox);
-            if (c.phiflag)
-                hipLaunchKernelGGL((kg_momentum<1, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, kz);
+            if (gl.phiflag)
+                hipLaunchKernelGGL((kg_momentum<1, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, gg);
             else
-                hipLaunchKernelGGL((kg_momentum<0, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, kz);
+                hipLaunchKernelGGL((kg_momentum<0, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, gg);
         }
     }
-    if (plan_regs(pl)) {
-        const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl);
-        const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl);
-        const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl);
-        const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl);
-        const size_t shx = sizeof(double2) * (size_t)xp * rline_stride(n);
-        with_log2_gang(reg_log2(pl), [&](auto LC) {
-            constexpr int L = decltype(LC)::value;
-            if (c.mixed)
-                hipLaunchKernelGGL((kg_xfwd<float, true, L>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g,
-                                   pl, xp, ld);
+    for (int q = 0; q < gl.nsize; q++) {
+        const GangSize &z = gl.size[q];
+        const GangBox *gs = g + z.off;
+        const int nb = z.cnt, n = z.n, nh = n / 2 + 1, pairs = n * n / 2, ld = z.ld;
+        const FftPlan &pl = z.plan;
+        const Tuning &tu = z.tune;
+        if (plan_regs(pl)) {
+            const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : rpairs(pl);
+            const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : rpairs(pl);
+            const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : rtile(pl);
+            const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : rtile(pl);
+            const size_t shx = sizeof(double2) * (size_t)xp * rline_stride(n);
+            with_log2_gang(reg_log2(pl), [&](auto LC) {
+                constexpr int L = decltype(LC)::value;
+                if (gl.mixed)
+                    hipLaunchKernelGGL((kg_xfwd<float, true, L>), dim3(pairs / xp, nb), dim3(xp * (n / 8)), shx, s, gs,
+                                       pl, xp, ld);
+                else
+                    hipLaunchKernelGGL((kg_xfwd<double, true, L>), dim3(pairs / xp, nb), dim3(xp * (n / 8)), shx, s,
+                                       gs, pl, xp, ld);
+            });
+            const dim3 gy((nh + ty - 1) / ty, n, nb), gz((nh + tz - 1) / tz, n, nb);
+            const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
+            with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
+                hipLaunchKernelGGL((kg_ypass<-1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, gs, pl,
+                                   ld, ty);
+            });
+            with_log2_gang(reg_log2_cols(pl, tz), [&](auto LC) {
+                hipLaunchKernelGGL((kg_zpoisson<true, decltype(LC)::value>), gz, dim3(tz * (n / 8)), shz, s, gs, pl,
+                                   ld, tz);
+            });
+            with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
+                hipLaunchKernelGGL((kg_ypass<1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, gs, pl,
+                                   ld, ty);
+            });
+            with_log2_gang(reg_log2(pl), [&](auto LC) {
+                hipLaunchKernelGGL((kg_xinv<true, decltype(LC)::value>), dim3(pairs / xip, nb), dim3(xip * (n / 8)),
+                                   sizeof(double2) * (size_t)xip * rline_stride(n), s, gs, pl, xip, ld);
+            });
+        } else {
+            const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : pairs_per_block(pl);
+            const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : pairs_per_block(pl);
+            const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : col_tile(pl);
+            const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
+            const size_t shx = sizeof(double2) * (size_t)xp * (n + 1);
+            if (gl.mixed)
+                hipLaunchKernelGGL((kg_xfwd<float, false, 0>), dim3(pairs / xp, nb), dim3(fft_threads(pl, xp)), shx,
+                                   s, gs, pl, xp, ld);
             else
-                hipLaunchKernelGGL((kg_xfwd<double, true, L>), dim3(pairs / xp, nbox), dim3(xp * (n / 8)), shx, s, g,
-                                   pl, xp, ld);
-        });
-        const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
-        const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
-        with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
-            hipLaunchKernelGGL((kg_ypass<-1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld,
-                               ty);
-        });
-        with_log2_gang(reg_log2_cols(pl, tz), [&](auto LC) {
-            hipLaunchKernelGGL((kg_zpoisson<true, decltype(LC)::value>), gz, dim3(tz * (n / 8)), shz, s, g, pl, ld,
-                               tz);
-        });
-        with_log2_gang(reg_log2_cols(pl, ty), [&](auto LC) {
-            hipLaunchKernelGGL((kg_ypass<1, true, decltype(LC)::value>), gy, dim3(ty * (n / 8)), shy, s, g, pl, ld,
-                               ty);
-        });
-        with_log2_gang(reg_log2(pl), [&](auto LC) {
-            hipLaunchKernelGGL((kg_xinv<true, decltype(LC)::value>), dim3(pairs / xip, nbox), dim3(xip * (n / 8)),
-                               sizeof(double2) * (size_t)xip * rline_stride(n), s, g, pl, xip, ld);
-        });
-    } else {
-        const int xp = tu.v[KNOB_XPAIRS] ? tu.v[KNOB_XPAIRS] : pairs_per_block(pl);
-        const int xip = tu.v[KNOB_XIPAIRS] ? tu.v[KNOB_XIPAIRS] : pairs_per_block(pl);
-        const int ty = tu.v[KNOB_YTILE] ? tu.v[KNOB_YTILE] : col_tile(pl);
-        const int tz = tu.v[KNOB_ZTILE] ? tu.v[KNOB_ZTILE] : col_tile(pl);
-        const size_t shx = sizeof(double2) * (size_t)xp * (n + 1);
-        if (c.mixed)
-            hipLaunchKernelGGL((kg_xfwd<float, false, 0>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
-                               pl, xp, ld);
-        else
-            hipLaunchKernelGGL((kg_xfwd<double, false, 0>), dim3(pairs / xp, nbox), dim3(fft_threads(pl, xp)), shx, s, g,
-                               pl, xp, ld);
-        const dim3 gy((nh + ty - 1) / ty, n, nbox), gz((nh + tz - 1) / tz, n, nbox);
-        const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
-        hipLaunchKernelGGL((kg_ypass<-1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL((kg_zpoisson<false, 0>), gz, dim3(fft_threads(pl, tz)), shz, s, g, pl, ld, tz);
-        hipLaunchKernelGGL((kg_ypass<1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, g, pl, ld, ty);
-        hipLaunchKernelGGL((kg_xinv<false, 0>), dim3(pairs / xip, nbox), dim3(fft_threads(pl, xip)),
-                           sizeof(double2) * (size_t)xip * (n + 1), s, g, pl, xip, ld);
+                hipLaunchKernelGGL((kg_xfwd<double, false, 0>), dim3(pairs / xp, nb), dim3(fft_threads(pl, xp)), shx,
+                                   s, gs, pl, xp, ld);
+            const dim3 gy((nh + ty - 1) / ty, n, nb), gz((nh + tz - 1) / tz, n, nb);
+            const size_t shy = sizeof(double2) * (size_t)ty * n, shz = sizeof(double2) * (size_t)tz * n;
+            hipLaunchKernelGGL((kg_ypass<-1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, gs, pl, ld, ty);
+            hipLaunchKernelGGL((kg_zpoisson<false, 0>), gz, dim3(fft_threads(pl, tz)), shz, s, gs, pl, ld, tz);
+            hipLaunchKernelGGL((kg_ypass<1, false, 0>), gy, dim3(fft_threads(pl, ty)), shy, s, gs, pl, ld, ty);
+            hipLaunchKernelGGL((kg_xinv<false, 0>), dim3(pairs / xip, nb), dim3(fft_threads(pl, xip)),
+                               sizeof(double2) * (size_t)xip * (n + 1), s, gs, pl, xip, ld);
+        }
     }
-    const dim3 gp((unsigned)nblocks((size_t)n * n * 64, BLK), nbox);
-    if (c.mixed)
-        hipLaunchKernelGGL(kg_project<float>, gp, dim3(BLK), 0, s, g);
+    const GangGrid &gp = gl.grid[GP_PROJECT], &gr = gl.grid[GP_PLANES];
+    if (gl.mixed)
+        hipLaunchKernelGGL(kg_project<float>, dim3((unsigned)gp.first[nbox]), dim3(BLK), 0, s, g, nbox, gp);
     else
-        hipLaunchKernelGGL(kg_project<double>, gp, dim3(BLK), 0, s, g);
-    hipLaunchKernelGGL(kg_reduce_planes, dim3(n, 3, nbox), dim3(64), 0, s, g);
+        hipLaunchKernelGGL(kg_project<double>, dim3((unsigned)gp.first[nbox]), dim3(BLK), 0, s, g, nbox, gp);
+    hipLaunchKernelGGL(kg_reduce_planes, dim3((unsigned)gr.first[nbox]), dim3(64), 0, s, g, nbox, gr);
     hipLaunchKernelGGL(kg_finalize, dim3(nbox), dim3(64), 0, s, g);
 }
 

@@ -776,9 +776,10 @@ extern "C" int mhit_step(mhit_ctx *c, int nsteps, mhit_diag *d)
     return s1.halted ? s1.status : MHIT_OK;
 }
 
-// Gang stepping: one launch per pass for several boxes of one class.  The
-// descriptors of both phi orientations (as now, and swapped) go to the lead
-// context's device buffer once per call; step s reads orientation s % 2.
+// Gang stepping: boxes of one precision and phiflag, of any sizes, stepped by
+// shared launches (the FFT passes once per size).  The descriptors of both phi
+// orientations (as now, and swapped) go to the lead context's device buffer
+// once per call, sorted by n; step s reads orientation s % 2.
 static_assert(GANG_MAX == MHIT_BATCH_MAX, "gang size");
 
 extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_diag *d)
@@ -787,7 +788,7 @@ extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_
     for (int b = 0; b < nbox; b++) {
         if (!ctx[b]) return MHIT_ERR_ARG;
         const Consts &k = ctx[b]->kc, &k0 = ctx[0]->kc;
-        if (k.n != k0.n || k.mixed != k0.mixed || k.phiflag != k0.phiflag || ctx[b]->device != ctx[0]->device)
+        if (k.mixed != k0.mixed || k.phiflag != k0.phiflag || ctx[b]->device != ctx[0]->device)
             return MHIT_ERR_ARG;
         for (int q = 0; q < b; q++)
             if (ctx[q] == ctx[b]) return MHIT_ERR_ARG;
@@ -800,23 +801,38 @@ extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_
         if (rc) return rc;
         lead->gang = (GangBox *)g;
     }
-    std::vector<mhit_ctx *> run;                  // boxes that step
+    std::vector<mhit_ctx *> run;                  // boxes that step, in the caller's order
     std::vector<Scalars> s0(nbox);
-    std::vector<GangBox> desc(2 * GANG_MAX);
     for (int b = 0; b < nbox; b++) {
         mhit_ctx *c = ctx[b];
         HIPCHK(hipStreamSynchronize(c->stream));  // its own earlier work
         if (c->halted) continue;
         int rc = read_scalars(c, &s0[b]);
         if (rc) return rc;
-        const int m = (int)run.size();
-        desc[m].f = c->f;
-        desc[m].c = c->kc;
-        desc[GANG_MAX + m] = desc[m];
-        std::swap(desc[GANG_MAX + m].f.phi, desc[GANG_MAX + m].f.phin);
         run.push_back(c);
     }
     const int m = (int)run.size();
+    // descriptor j is box order[j]: sorted by n, so each size's boxes are
+    // consecutive for its FFT launches
+    std::vector<int> order(m);
+    for (int q = 0; q < m; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return run[a]->kc.n < run[b]->kc.n; });
+    std::vector<GangBox> desc(2 * GANG_MAX);
+    std::vector<GangSize> sizes(GANG_MAX);
+    GangLaunch gl;
+    for (int j = 0; j < m; j++) {
+        const mhit_ctx *c = run[order[j]];
+        desc[j].f = c->f;
+        desc[j].c = c->kc;
+        sizes[j].ld = c->f.ld;
+        sizes[j].plan = c->plan;
+        sizes[j].tune = c->tune;
+    }
+    if (m > 0) gang_layout(desc.data(), m, sizes.data(), &gl);
+    for (int j = 0; j < m; j++) {
+        desc[GANG_MAX + j] = desc[j];
+        std::swap(desc[GANG_MAX + j].f.phi, desc[GANG_MAX + j].f.phin);
+    }
     std::vector<int> since0(m);
     if (m > 0) {
         hipStream_t s = lead->stream;
@@ -825,13 +841,12 @@ extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_
                               hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         unsigned euler = 0;
-        for (int q = 0; q < m; q++) {
-            since0[q] = run[q]->nsteps_since_init;
-            if (since0[q] == 0) euler |= 1u << q;
-        }
+        for (int q = 0; q < m; q++) since0[q] = run[q]->nsteps_since_init;
+        for (int j = 0; j < m; j++)
+            if (since0[order[j]] == 0) euler |= 1u << j;
         for (int st = 0; st < nsteps; st++) {
             const GangBox *g = lead->gang + (run[0]->kc.phiflag && (st & 1) ? GANG_MAX : 0);
-            launch_gang_step(g, m, st == 0 ? euler : 0u, lead->kc, lead->plan, lead->f.ld, lead->tune, s);
+            launch_gang_step(g, m, st == 0 ? euler : 0u, gl, s);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
@@ -1472,6 +1487,10 @@ struct Job {
 
 // Boxes of one (nx, precision, phiflag) stepped together: one launch per pass
 // for all members (continuous batching: boxes join and leave between steps).
+// The launches are those of mhit_step_batch, which also takes boxes of several
+// sizes; the ensemble keeps one gang per size, each on its own stream, which
+// measured faster on streams of mixed sizes than one gang whose FFT passes run
+// one size after the other.
 // Each change of membership uploads a fresh descriptor set into the next slot
 // of a ring in device memory; a slot is reused only after the event recorded
 // behind its last launch has completed.
@@ -1480,6 +1499,7 @@ struct EnsGang {
     int nx = 0, mixed = 0, phiflag = 0;
     hipStream_t stream = nullptr;
     std::vector<EnsReq *> members;       // boxes that still take steps, descriptor order
+    GangLaunch gl{};                     // launches of the uploaded membership
     bool dirty = true;                   // members changed since the last upload
     GangBox *dev = nullptr;              // GANG_RING x (2 x GANG_MAX) descriptors
     GangBox *host = nullptr;             // pinned staging, same layout
@@ -1739,9 +1759,11 @@ static int gang_upload(EnsGang *g)
             x = nullptr;
         }
     GangBox *h = g->host + (size_t)s * 2 * GANG_MAX;
-    for (size_t b = 0; b < g->members.size(); b++) {
+    GangSize sizes[GANG_MAX];
+    const int m = (int)g->members.size();
+    for (int b = 0; b < m; b++) {
         const EnsReq *r = g->members[b];
-        GangBox &d0 = h[b], &d1 = h[GANG_MAX + b];
+        GangBox &d0 = h[b];
         d0.f = r->ctx->f;
         d0.c = r->ctx->kc;
         if (g->phiflag) {
@@ -1749,7 +1771,14 @@ static int gang_upload(EnsGang *g)
             d0.f.phi = odd ? r->phin0 : r->phi0;
             d0.f.phin = odd ? r->phi0 : r->phin0;
         }
-        d1 = d0;
+        sizes[b].ld = r->ctx->f.ld;
+        sizes[b].plan = r->ctx->plan;
+        sizes[b].tune = r->ctx->tune;
+    }
+    gang_layout(h, m, sizes, &g->gl);
+    for (int b = 0; b < m; b++) {
+        GangBox &d1 = h[GANG_MAX + b];
+        d1 = h[b];
         std::swap(d1.f.phi, d1.f.phin);
     }
     HIPCHK(hipMemcpyAsync(g->dev + (size_t)s * 2 * GANG_MAX, h, sizeof(GangBox) * 2 * GANG_MAX,
@@ -1767,12 +1796,11 @@ static int gang_upload(EnsGang *g)
 constexpr int GANG_STEADY = 4;
 static int gang_launch(mhit_ens *e, EnsGang *g, unsigned euler)
 {
-    const mhit_ctx *c0 = g->members[0]->ctx;
     const int par = g->phiflag ? g->parity : 0;     // without phi both sets are the same
     const GangBox *d = g->dev + (size_t)g->slot * 2 * GANG_MAX + (par ? GANG_MAX : 0);
     const int m = (int)g->members.size();
     if (euler || !e->cfg.use_graphs || g->steady < GANG_STEADY) {
-        launch_gang_step(d, m, euler, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
+        launch_gang_step(d, m, euler, g->gl, g->stream);
         HIPCHK(hipGetLastError());
         return MHIT_OK;
     }
@@ -1780,7 +1808,7 @@ static int gang_launch(mhit_ens *e, EnsGang *g, unsigned euler)
     if (!x) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
-        launch_gang_step(d, m, 0u, c0->kc, c0->plan, c0->f.ld, c0->tune, g->stream);
+        launch_gang_step(d, m, 0u, g->gl, g->stream);
         HIPCHK(hipStreamEndCapture(g->stream, &gr));
         hipError_t ei = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);

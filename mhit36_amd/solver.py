@@ -224,9 +224,10 @@ class Solver:
 
 
 def step_batch(solvers, nsteps=1, raise_on_stop=True):
-    """Advance several boxes of one nx, precision and phiflag together: one
-    launch per pass for all of them (mhit_step_batch).  Each box ends
-    bit-identical to Solver.step on it alone.  Returns one Diag per box."""
+    """Advance several boxes of one precision and phiflag, of one or several
+    sizes, together: shared launches for all of them, the FFT passes one per
+    size (mhit_step_batch).  Each box ends bit-identical to Solver.step on it
+    alone.  Returns one Diag per box."""
     solvers = list(solvers)
     if not 1 <= len(solvers) <= MHIT_BATCH_MAX:
         raise ValueError(f"1..{MHIT_BATCH_MAX} boxes per batch, got {len(solvers)}")
