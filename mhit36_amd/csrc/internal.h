@@ -173,9 +173,21 @@ struct GangLaunch {
 // The launch plan gl of the nbox descriptors d (sorted by n, one precision and
 // phiflag); sizes[q] gives the plan, ld and shapes of descriptor q's size.
 void gang_layout(const GangBox *d, int nbox, const GangSize *sizes, GangLaunch *gl);
+// Streams and events that let the FFT passes of a gang's sizes run side by
+// side: size q > 0 on side[q-1], forked from and joined back to the gang's
+// stream (default priority, non-blocking).
+struct GangStreams {
+    int n = 0;                           // side streams made
+    hipStream_t side[GANG_MAX - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[GANG_MAX - 1] = {};
+};
 // One step of the gang (descriptors g in device memory, laid out by
-// gang_layout; bit b of euler: box b takes its Euler first step).
-void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s);
+// gang_layout; bit b of euler: box b takes its Euler first step) on stream s;
+// with fs holding a side stream per size beyond the first, the sizes' FFT
+// passes run on their own streams between a fork and a join.
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s,
+                      const GangStreams *fs = nullptr);
 void launch_debug_math(int op, size_t count, const double *x, double *y, const double *mtab, hipStream_t s);
 // Drop counting: label the phi >= 0.5 cells (26-connected, periodic) by the
 // smallest linear index of their set; list the roots; count cells per root

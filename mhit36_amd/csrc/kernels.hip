@@ -2752,8 +2752,11 @@ void gang_layout(const GangBox *d, int nbox, const GangSize *sizes, GangLaunch *
 // One step of a gang laid out by gang_layout (device descriptors g[0..nbox)),
 // with the launch shapes of the one-box passes above: the shared passes once,
 // the FFT passes once per size, on that size's boxes.
-void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s)
+void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaunch &gl, hipStream_t s0,
+                      const GangStreams *fs)
 {
+    const bool fork = fs && gl.nsize > 1 && fs->n >= gl.nsize - 1;
+    hipStream_t s = s0;
     if (gl.phiflag) {
         const GangGrid &gg = gl.grid[GP_ACDI];
         const dim3 grid((unsigned)gg.first[nbox]);
@@ -2777,7 +2780,10 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaun
                 hipLaunchKernelGGL((kg_momentum<0, double>), grid, dim3(MTHREADS), 0, s, g, nbox, euler, gg);
         }
     }
+    if (fork) (void)hipEventRecord(fs->fork, s0);
     for (int q = 0; q < gl.nsize; q++) {
+        s = fork && q > 0 ? fs->side[q - 1] : s0;
+        if (fork && q > 0) (void)hipStreamWaitEvent(s, fs->fork, 0);
         const GangSize &z = gl.size[q];
         const GangBox *gs = g + z.off;
         const int nb = z.cnt, n = z.n, nh = n / 2 + 1, pairs = n * n / 2, ld = z.ld;
@@ -2836,7 +2842,11 @@ void launch_gang_step(const GangBox *g, int nbox, unsigned euler, const GangLaun
             hipLaunchKernelGGL((kg_xinv<false, 0>), dim3(pairs / xip, nb), dim3(fft_threads(pl, xip)),
                                sizeof(double2) * (size_t)xip * (n + 1), s, gs, pl, xip, ld);
         }
+        if (fork && q > 0) (void)hipEventRecord(fs->join[q - 1], s);
     }
+    s = s0;
+    if (fork)
+        for (int q = 1; q < gl.nsize; q++) (void)hipStreamWaitEvent(s0, fs->join[q - 1], 0);
     const GangGrid &gp = gl.grid[GP_PROJECT], &gr = gl.grid[GP_PLANES];
     if (gl.mixed)
         hipLaunchKernelGGL(kg_project<float>, dim3((unsigned)gp.first[nbox]), dim3(BLK), 0, s, g, nbox, gp);

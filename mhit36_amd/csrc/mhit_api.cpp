@@ -110,6 +110,7 @@ struct mhit_ctx {
     std::vector<PendingTiming> pending;
     std::map<std::string, KStat> stats;
     GangBox *gang = nullptr;     // device descriptors of mhit_step_batch (2 x GANG_MAX)
+    GangStreams gang_fs;         // side streams of mhit_step_batch's per-size FFT passes
     void *ccl_ws = nullptr;      // drop-count workspace, made by the first mhit_drop_count
     size_t ccl_bytes = 0;
 };
@@ -284,6 +285,13 @@ static void free_all(mhit_ctx *c)
     c->ccl_bytes = 0;
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     c->event_pool.clear();
+    for (int q = 0; q < c->gang_fs.n; q++) {
+        (void)hipStreamSynchronize(c->gang_fs.side[q]);
+        (void)hipStreamDestroy(c->gang_fs.side[q]);
+        (void)hipEventDestroy(c->gang_fs.join[q]);
+    }
+    if (c->gang_fs.fork) (void)hipEventDestroy(c->gang_fs.fork);
+    c->gang_fs = GangStreams{};
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -829,6 +837,19 @@ extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_
         sizes[j].tune = c->tune;
     }
     if (m > 0) gang_layout(desc.data(), m, sizes.data(), &gl);
+    // a side stream for each size beyond the first, kept by the lead context
+    if (m > 0 && gl.nsize > 1) {
+        GangStreams &fs = lead->gang_fs;
+        if (!fs.fork) HIPCHK(hipEventCreateWithFlags(&fs.fork, hipEventDisableTiming));
+        while (fs.n < gl.nsize - 1) {
+            HIPCHK(hipStreamCreateWithFlags(&fs.side[fs.n], hipStreamNonBlocking));
+            if (hipEventCreateWithFlags(&fs.join[fs.n], hipEventDisableTiming) != hipSuccess) {
+                (void)hipStreamDestroy(fs.side[fs.n]);
+                return MHIT_ERR_HIP;
+            }
+            fs.n++;
+        }
+    }
     for (int j = 0; j < m; j++) {
         desc[GANG_MAX + j] = desc[j];
         std::swap(desc[GANG_MAX + j].f.phi, desc[GANG_MAX + j].f.phin);
@@ -846,7 +867,7 @@ extern "C" int mhit_step_batch(mhit_ctx *const *ctx, int nbox, int nsteps, mhit_
             if (since0[order[j]] == 0) euler |= 1u << j;
         for (int st = 0; st < nsteps; st++) {
             const GangBox *g = lead->gang + (run[0]->kc.phiflag && (st & 1) ? GANG_MAX : 0);
-            launch_gang_step(g, m, st == 0 ? euler : 0u, gl, s);
+            launch_gang_step(g, m, st == 0 ? euler : 0u, gl, s, &lead->gang_fs);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
