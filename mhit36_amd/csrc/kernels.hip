@@ -1635,21 +1635,82 @@ __device__ __forceinline__ void neumaier(double &s, double &c, double x)
     s = t;
 }
 
+// The value of lane (this lane XOR OFF) of the wavefront, without a trip
+// through the LDS crossbar where the hardware has a direct form: the gfx950
+// lane-half swaps for 32 and 16 (each returns both halves; a lane keeps the
+// other half's value), DPP row_ror:8 for 8, quad permutes for 2 and 1, and a
+// swizzle for 4.
+template <int OFF>
+__device__ __forceinline__ unsigned lane_xor32(unsigned x)
+{
+    const unsigned l = threadIdx.x & 63;
+    if constexpr (OFF == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return l < 32 ? p[1] : p[0];
+    } else if constexpr (OFF == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (l & 16) ? p[0] : p[1];
+    } else if constexpr (OFF == 8) {
+        return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xf, 0xf, false);   // row_ror:8
+    } else if constexpr (OFF == 4) {
+        return (unsigned)__builtin_amdgcn_ds_swizzle((int)x, (4 << 10) | 0x1f);            // xor 4
+    } else if constexpr (OFF == 2) {
+        return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4e, 0xf, 0xf, false);    // quad_perm [2,3,0,1]
+    } else {
+        static_assert(OFF == 1, "lane_xor: 32, 16, 8, 4, 2 or 1");
+        return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xb1, 0xf, 0xf, false);    // quad_perm [1,0,3,2]
+    }
+}
+template <int OFF>
+__device__ __forceinline__ double lane_xor(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = lane_xor32<OFF>((unsigned)b), hi = lane_xor32<OFF>((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum and max over the 64 lanes in the xor-butterfly order 32, 16, ..., 1
+// (every lane ends with the result).  DIRECT: through lane_xor, the form for
+// reductions on the critical path (a projection line of 64 points, the plane
+// and total sums); otherwise through the LDS crossbar (__shfl_xor), fewer
+// VALU instructions where a lane has several points to add first.
+template <bool DIRECT = true>
 __device__ __forceinline__ double wave_sum(double x)
 {
+    if constexpr (DIRECT) {
+        x = x + lane_xor<32>(x);
+        x = x + lane_xor<16>(x);
+        x = x + lane_xor<8>(x);
+        x = x + lane_xor<4>(x);
+        x = x + lane_xor<2>(x);
+        x = x + lane_xor<1>(x);
+    } else {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x = x + __shfl_xor(x, off, 64);
+        for (int off = 32; off >= 1; off >>= 1) x = x + __shfl_xor(x, off, 64);
+    }
     return x;
 }
 
+template <bool DIRECT = true>
 __device__ __forceinline__ double wave_max(double x)
 {
+    if constexpr (DIRECT) {
+        x = nanmax(x, lane_xor<32>(x));
+        x = nanmax(x, lane_xor<16>(x));
+        x = nanmax(x, lane_xor<8>(x));
+        x = nanmax(x, lane_xor<4>(x));
+        x = nanmax(x, lane_xor<2>(x));
+        x = nanmax(x, lane_xor<1>(x));
+    } else {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x = nanmax(x, __shfl_xor(x, off, 64));
+        for (int off = 32; off >= 1; off >>= 1) x = nanmax(x, __shfl_xor(x, off, 64));
+    }
     return x;
 }
 
-template <typename T>
+// DIRECT: the line sums' lane exchanges through lane_xor (small n, where a
+// lane holds one or two points of its line and the reduction is the work).
+template <typename T, bool DIRECT>
 __device__ __forceinline__ void k_project_body(uint3 blk,
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     const double *__restrict__ p, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
@@ -1685,8 +1746,8 @@ __device__ __forceinline__ void k_project_body(uint3 blk,
         mu_ = nanmax(mu_, uu); mv = nanmax(mv, vv); mw = nanmax(mw, ww);
     }
     if constexpr (sizeof(T) == 4) { su = su + cu; sv = sv + cv; sw = sw + cw; }
-    su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
-    mu_ = wave_max(mu_); mv = wave_max(mv); mw = wave_max(mw);
+    su = wave_sum<DIRECT>(su); sv = wave_sum<DIRECT>(sv); sw = wave_sum<DIRECT>(sw);
+    mu_ = wave_max<DIRECT>(mu_); mv = wave_max<DIRECT>(mv); mw = wave_max<DIRECT>(mw);
     if (lane == 0) {
         const size_t nn = (size_t)n * n;
         lsum[line] = su; lsum[nn + line] = sv; lsum[2 * nn + line] = sw;
@@ -1694,13 +1755,14 @@ __device__ __forceinline__ void k_project_body(uint3 blk,
     }
 }
 
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ void __launch_bounds__(BLK) k_project(
     const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
     const double *__restrict__ p, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
     double *__restrict__ lsum, double *__restrict__ lmax, const Scalars *__restrict__ sc, Consts c)
 {
-    k_project_body<T>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, p, u, v, w, lsum, lmax, sc, c);
+    k_project_body<T, DIRECT>(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), us, vs, ws, p, u, v, w, lsum, lmax,
+                              sc, c);
 }
 
 
@@ -1920,7 +1982,7 @@ __global__ void __launch_bounds__(BLK) kg_project(const GangBox *__restrict__ g,
 {
     unsigned l;
     const GangBox &B = g[gang_block(gg, nbox, l)];
-    k_project_body<T>(make_uint3(l, 0, 0), (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.p,
+    k_project_body<T, true>(make_uint3(l, 0, 0), (const T *)B.f.us, (const T *)B.f.vs, (const T *)B.f.ws, B.f.p,
                       (T *)B.f.u, (T *)B.f.v, (T *)B.f.w, B.f.lsum, B.f.lmax, B.f.sc, B.c);
 }
 
@@ -2695,13 +2757,19 @@ void launch_xinv(const Fields &f, const Consts &c, const FftPlan &pl, const Tuni
 void launch_project(const Fields &f, const Consts &c, hipStream_t s, LaunchInfo *li)
 {
     size_t threads = (size_t)c.n * c.n * 64;
-    if (c.mixed)
-        hipLaunchKernelGGL(k_project<float>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, (const float *)f.us,
-                           (const float *)f.vs, (const float *)f.ws, f.p, (float *)f.u, (float *)f.v, (float *)f.w,
-                           f.lsum, f.lmax, f.sc, c);
-    else
-        hipLaunchKernelGGL(k_project<double>, dim3(nblocks(threads, BLK)), dim3(BLK), 0, s, f.us, f.vs, f.ws, f.p,
-                           f.u, f.v, f.w, f.lsum, f.lmax, f.sc, c);
+    // lines of up to 128 points: the line sums' exchanges are the work, so the
+    // direct lane exchanges (for 8 points per lane at 512 the crossbar form
+    // measured 2% faster)
+    const bool direct = c.n <= 128;
+    const dim3 grid(nblocks(threads, BLK));
+    if (c.mixed) {
+        auto kern = direct ? k_project<float, true> : k_project<float, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(BLK), 0, s, (const float *)f.us, (const float *)f.vs, (const float *)f.ws,
+                           f.p, (float *)f.u, (float *)f.v, (float *)f.w, f.lsum, f.lmax, f.sc, c);
+    } else {
+        auto kern = direct ? k_project<double, true> : k_project<double, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(BLK), 0, s, f.us, f.vs, f.ws, f.p, f.u, f.v, f.w, f.lsum, f.lmax, f.sc, c);
+    }
     li->name = "project"; li->bytes = (c.mixed ? 4.0 : 7.0) * F(c);
 }
 
