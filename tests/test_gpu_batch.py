@@ -105,6 +105,13 @@ def test_gang_of_uneven_sizes(phiflag, precision):
     check_gang([64, 32, 128, 48, 64, 96], phiflag, precision, nbox=6, nsteps=4, forced=True)
 
 
+@pytest.mark.parametrize("phiflag,precision", [(1, 0), (0, 1)])
+def test_gang_with_large_boxes(phiflag, precision):
+    """A 256^3 box (register FFT with the plan read at run time, 8-column
+    tiles) and a 192^3 box (radix-3, LDS-staged) in one gang with a 64^3 box."""
+    check_gang([256, 64, 192], phiflag, precision, nbox=3, nsteps=2, forced=True)
+
+
 def test_gang_of_uneven_sizes_mixes_euler_and_ab2_steps():
     check_gang([48, 32, 64, 32], 1, 0, nbox=4, nsteps=3, pre=(2, 0, 1, 3))
 
