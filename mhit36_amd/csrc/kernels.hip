@@ -124,7 +124,11 @@ __device__ __forceinline__ T *at_off(T *base, unsigned off)
     return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off * (unsigned)sizeof(T));
 }
 
-template <typename T>
+// MARCH3: the march unrolled by three planes (the one-box FP64 pass); else
+// one plane per iteration with the registers copied forward (FP32, and the
+// gang pass, whose per-box addressing would not fit the unrolled march in 128
+// VGPRs).
+template <typename T, bool MARCH3 = (sizeof(T) == 8)>
 __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, const double *__restrict__ mtab,
@@ -190,59 +194,65 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
         __syncthreads();
     }
 
-    // Column registers: phi of the own cell (f*) and of the ring cell (hf*)
-    // for the planes not yet in LDS, the z flux of the own cell (z*); suffix
-    // = plane - k.  psi(k), psi(k+1) and phi(k-1), phi(k) of a thread's own
-    // cells are read back from the LDS slots it wrote itself.
-    T f_p1, f_p2, fpre = 0;
-    T z_m1, z_0;
-    T hf_p1 = 0, hf_p2 = 0, hfpre = 0;
-    T nu0 = 0, nuxp = 0, nv0 = 0, nvyp = 0, nw0 = 0;   // u, v, w of the output plane
-    T nwp = 0;                                         // w of the plane above it
-    T pu0 = 0, puxp = 0, pv0 = 0, pvyp = 0, pwp = 0;   // the same one plane up, in flight
+    // Column registers of the march.  It is unrolled by three planes, and each
+    // register's role (plane - k) is a compile-time index into a 3-element
+    // array, so planes rotate through the registers without copies: a copy of
+    // a register that a load is still filling would wait for that load, and
+    // cap every load's lead at the end of the iteration that issued it.
+    //   fr: phi of the own cell, hr: of the ring cell (planes not yet in LDS);
+    //   zr: z flux of the own cell; U: u, u(x+1), v, v(y+1) of a plane and w
+    //   of the plane above it (two planes live: the output plane and the next,
+    //   in flight); wk: w of the output plane.
+    // psi(k), psi(k+1) and phi(k-1), phi(k) of a thread's own cells are read
+    // back from the LDS slots it wrote itself.  Every thread issues every load
+    // of a plane (the offsets wrap, so a thread outside the box or without a
+    // ring cell reads a valid cell it does not use): the count of loads per
+    // plane is then fixed, and a wait for one plane's data leaves the later
+    // planes' loads in flight.
+    T fr[3] = {0, 0, 0}, hr[3] = {0, 0, 0}, zr[3] = {0, 0, 0};
+    T U[3][5] = {};
+    T wk = 0;
+    const int kend = min(k0 + kz, n);
     // the x+1 and y+1 neighbours of the own cell, as offsets from it (periodic)
     const unsigned xpo = (unsigned)wrap(i + 1, n) + (unsigned)n * (unsigned)wrap(j, n);
     const unsigned ypo = (unsigned)wrap(i, n) + (unsigned)n * (unsigned)wrap(j + 1, n);
-    // u, v of plane m and w of plane m+1 into the in-flight registers
-    auto fetch_uvw = [&](int m) {
-        if (active) {
-            // the plane offsets are opaque: otherwise the compiler hoists
-            // base + own out of the march as 64-bit vector pointers, which
-            // spill
-            size_t z = nn * (size_t)m, zp = nn * (size_t)(m + 1 == n ? 0 : m + 1);
-            asm volatile("" : "+s"(z), "+s"(zp));
-            const T *up = u + z, *vp = v + z;
-            pu0 = ld_off(up, own); puxp = ld_off(up, xpo);
-            pv0 = ld_off(vp, own); pvyp = ld_off(vp, ypo);
-            pwp = ld_off(w + zp, own);
-        }
+    // u, v of plane m < n and w of plane m+1 into the registers of one plane
+    auto fetch_uvw = [&](int m, T (&d)[5]) {
+        // the plane offsets are opaque: otherwise the compiler hoists base +
+        // own out of the march as 64-bit vector pointers, which spill
+        size_t z = nn * (size_t)m, zp = nn * (size_t)(m + 1 == n ? 0 : m + 1);
+        asm volatile("" : "+s"(z), "+s"(zp));
+        const T *up = u + z, *vp = v + z;
+        d[0] = ld_off(up, own); d[1] = ld_off(up, xpo);
+        d[2] = ld_off(vp, own); d[3] = ld_off(vp, ypo);
+        d[4] = ld_off(w + zp, own);
     };
 
     // prologue: psi(k0-1), psi(k0) in LDS -> z flux of k0-1, s(k0), phi(k0-1),
-    // phi(k0) in LDS -> psi(k0+1) in LDS.
+    // phi(k0) in LDS -> psi(k0+1) in LDS; the registers in the roles of the
+    // march's first plane.
     {
         const T fm2 = ld_phi(own, k0 - 2), fm1 = ld_phi(own, k0 - 1), f0 = ld_phi(own, k0);
-        f_p1 = ld_phi(own, k0 + 1); f_p2 = ld_phi(own, k0 + 2);
+        fr[1] = ld_phi(own, k0 + 1); fr[2] = ld_phi(own, k0 + 2);
         T hfm1 = 0, hf0 = 0;
         if (hk) {
-            hfm1 = ld_phi(hoff, k0 - 1); hf0 = ld_phi(hoff, k0); hf_p1 = ld_phi(hoff, k0 + 1);
-            hf_p2 = ld_phi(hoff, k0 + 2);
+            hfm1 = ld_phi(hoff, k0 - 1); hf0 = ld_phi(hoff, k0); hr[1] = ld_phi(hoff, k0 + 1);
+            hr[2] = ld_phi(hoff, k0 + 2);
         }
-        if (active) nwp = ld_off(w + nn * (size_t)k0, own);
-        fetch_uvw(k0);
-        nu0 = pu0; nuxp = puxp; nv0 = pv0; nvyp = pvyp; nw0 = nwp; nwp = pwp;
-        const T qm2 = psi_of(fm2), qm1 = psi_of(fm1), q0 = psi_of(f0), qp1 = psi_of(f_p1);
+        wk = ld_off(w + nn * (size_t)k0, own);
+        fetch_uvw(k0, U[0]);
+        const T qm2 = psi_of(fm2), qm1 = psi_of(fm1), q0 = psi_of(f0), qp1 = psi_of(fr[1]);
         T hqm1 = 0, hq0 = 0, hqp1 = 0;
-        if (hk) { hqm1 = psi_of(hfm1); hq0 = psi_of(hf0); hqp1 = psi_of(hf_p1); }
+        if (hk) { hqm1 = psi_of(hfm1); hq0 = psi_of(hf0); hqp1 = psi_of(hr[1]); }
         const int sm1 = (k0 - 1) & 1, s0 = k0 & 1;
         psiL[sm1][pI] = qm1; psiL[s0][pI] = q0;
         if (hk) { psiL[sm1][hP] = hqm1; psiL[s0][hP] = hq0; }
         __syncthreads();
         T sx, sy;
         sharp(psiL[sm1][pI + 1] - psiL[sm1][pI - 1], psiL[sm1][pI + APX] - psiL[sm1][pI - APX], q0 - qm2, qm1,
-              fm1, sx, sy, z_m1);
+              fm1, sx, sy, zr[0]);
         sharp(psiL[s0][pI + 1] - psiL[s0][pI - 1], psiL[s0][pI + APX] - psiL[s0][pI - APX], qp1 - qm1, q0, f0,
-              sx, sy, z_0);
+              sx, sy, zr[1]);
         sxL[s0][sI] = sx; syL[s0][sI] = sy;
         phL[s0][sI] = f0; phL[sm1][sI] = fm1;
         if (hk == 1 || hk == 2) {
@@ -258,21 +268,30 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
         __syncthreads();
     }
 
-    const int kend = min(k0 + kz, n);
-    for (int k = k0; k < kend; k++) {
+    // Output plane k, the march's plane R0 of three.  Roles: fr/hr [R1] phi(k+1),
+    // [R2] phi(k+2), [R0] <- phi(k+3); zr [R0] flux(k-1), [R1] flux(k),
+    // [R2] <- flux(k+1); U[R0] plane k, U[R1] <- plane k+1.
+    auto plane = [&](auto PHC, int k) {
+        constexpr int R0 = decltype(PHC)::value, R1 = (R0 + 1) % 3, R2 = (R0 + 2) % 3;
+        const T f_p1 = fr[R1], f_p2 = fr[R2], hf_p1 = hr[R1], hf_p2 = hr[R2];
+        const T z_m1 = zr[R0];
         const int s0 = k & 1, s1 = (k + 1) & 1;
-        // psi(k+2), and the phi loads of the next iteration
+        // phi(k+3) (psi of the next plane waits for it), psi(k+2), then u, v, w
+        // of the next output plane (issued after psi: fewer registers live
+        // through it)
+        fr[R0] = ld_phi(own, k + 3);
+        hr[R0] = ld_phi(hoff, k + 3);
         const T q_p2 = psi_of(f_p2);
         T hq_p2 = 0;
         if (hk) hq_p2 = psi_of(hf_p2);
-        fpre = ld_phi(own, k + 3);
-        if (hk) hfpre = ld_phi(hoff, k + 3);
-        if (k + 1 < kend) fetch_uvw(k + 1);       // used by the next iteration's output
+        fetch_uvw(min(k + 1, n - 1), U[R1]);  // the next output plane (past the slab: unused)
+        const T nu0 = U[R0][0], nuxp = U[R0][1], nv0 = U[R0][2], nvyp = U[R0][3], nwp = U[R0][4], nw0 = wk;
         // s(k+1): x/y parts of the normal from psi(k+1) in LDS, z part from the
         // column (psi(k) from the own slot, psi(k+2) just formed)
         T sx, sy, z_p1;
         sharp(psiL[s1][pI + 1] - psiL[s1][pI - 1], psiL[s1][pI + APX] - psiL[s1][pI - APX], q_p2 - psiL[s0][pI],
               psiL[s1][pI], f_p1, sx, sy, z_p1);
+        zr[R2] = z_p1;
         sxL[s1][sI] = sx; syL[s1][sI] = sy;
         if (hk == 1 || hk == 2) {
             T hx, hy, hz;
@@ -320,24 +339,42 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
                 __builtin_nontemporal_store(f0 + (T)c.dt * r2, at_off(phin + nn * (size_t)k, own));
             }
         }
-        nu0 = pu0; nuxp = puxp; nv0 = pv0; nvyp = pvyp; nw0 = nwp; nwp = pwp;
-        // psi(k+2) and phi(k+1) for the x/y neighbours of the next iteration
+        // psi(k+2) and phi(k+1) for the x/y neighbours of the next plane
         psiL[s0][pI] = q_p2;
         phL[s1][sI] = f_p1;
         if (hk) psiL[s0][hP] = hq_p2;
         if (hk == 1 || hk == 2) phL[s1][hS] = hf_p1;
+        wk = nwp;
         __syncthreads();
-        f_p1 = f_p2; f_p2 = fpre;
-        z_m1 = z_0; z_0 = z_p1;
-        hf_p1 = hf_p2; hf_p2 = hfpre;
+    };
+    if constexpr (MARCH3) {
+        for (int k = k0; k < kend; k += 3) {
+            plane(std::integral_constant<int, 0>(), k);
+            if (k + 1 < kend) plane(std::integral_constant<int, 1>(), k + 1);
+            if (k + 2 < kend) plane(std::integral_constant<int, 2>(), k + 2);
+        }
+    } else {
+        // one plane per iteration with the roles fixed and the registers copied
+        // forward (FP32: within the 64 VGPRs of eight waves per SIMD, where
+        // the three-plane form spills)
+        for (int k = k0; k < kend; k++) {
+            plane(std::integral_constant<int, 0>(), k);
+            fr[1] = fr[2]; fr[2] = fr[0];
+            hr[1] = hr[2]; hr[2] = hr[0];
+            zr[0] = zr[1]; zr[1] = zr[2];
+#pragma unroll
+            for (int q = 0; q < 5; q++) U[0][q] = U[1][q];
+        }
     }
 }
 
-// FP32: 8 waves per SIMD (63 VGPRs).  FP64: 112 VGPRs, two blocks per CU;
-// held to the 80 VGPRs of 6 waves (three blocks, which the 44 KB of LDS would
-// allow) the pass spills and runs 2.3x slower.
+// FP32: 8 waves per SIMD (63 VGPRs).  FP64: held to the 128 VGPRs of 4 waves
+// per SIMD (two blocks per CU); its one spilled value belongs to the log's
+// branch for zero, negative or non-finite arguments.  Held to the 80 VGPRs of
+// 6 waves (three blocks, which the 44 KB of LDS would allow) the pass spills
+// in the march and runs 2.3x slower.
 template <typename T>
-__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) k_acdi(
+__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 4) k_acdi(
     const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
     const T *__restrict__ phi, T *__restrict__ phin, const Scalars *__restrict__ sc, const double *__restrict__ mtab,
     Consts c, int kz)
@@ -1882,15 +1919,15 @@ __device__ __forceinline__ uint3 tile_of(unsigned l, unsigned gx, unsigned gy)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 1) kg_acdi(const GangBox *__restrict__ g, int nbox,
+__global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 4) kg_acdi(const GangBox *__restrict__ g, int nbox,
                                                                            GangGrid gg)
 {
     unsigned l;
     const unsigned box = gang_block(gg, nbox, l);
     const GangBox &B = g[box];
     const unsigned n = (unsigned)B.c.n;
-    k_acdi_body<T>(tile_of(l, (n + ATX - 1) / ATX, (n + ATY - 1) / ATY), (const T *)B.f.u, (const T *)B.f.v,
-                   (const T *)B.f.w, (const T *)B.f.phi, (T *)B.f.phin, B.f.sc, B.f.mtab, B.c, gg.kz[box]);
+    k_acdi_body<T, false>(tile_of(l, (n + ATX - 1) / ATX, (n + ATY - 1) / ATY), (const T *)B.f.u, (const T *)B.f.v,
+                          (const T *)B.f.w, (const T *)B.f.phi, (T *)B.f.phin, B.f.sc, B.f.mtab, B.c, gg.kz[box]);
 }
 
 // euler: bit b set when box b takes its first (Euler) step, main.f90:323-324.
