@@ -34,7 +34,7 @@ __device__ __forceinline__ uint3 xcd_block()
 }
 
 // ---------------------------------------------------------------------------
-// Phase field, main.f90:338-486, fused into one pass (A4-A10).
+// Phase field, main.f90:338-486, fused into one pass.
 //
 // A block owns an ATX x ATY column of cells and marches kz planes in z; thread
 // t owns the cell (t % ATX, t / ATX) of the tile in every plane.  What a cell
@@ -42,9 +42,9 @@ __device__ __forceinline__ uint3 xcd_block()
 // at k-1..k+1, the z sharpening flux at k-1 and k+1) stays in the owning
 // thread's registers; only what x/y neighbours share goes through LDS, one
 // plane of each, double-buffered by plane parity:
-//   psi (A6) on the tile plus a 2-cell ring (the 12 outer corners are never
+//   psi on the tile plus a 2-cell ring (the 12 outer corners are never
 //   read), for the x/y parts of the normal;
-//   s_x, s_y of the sharpening flux s = a*n (A7-A9; a = 0.25(1 -
+//   s_x, s_y of the sharpening flux s = a*n (a = 0.25(1 -
 //   tanh^2(psi/(2 eps))), n the normalised undivided normal) on the tile plus
 //   a 1-cell ring without corners, for the flux divergence;
 //   phi on the same cells, for the convective and diffusive stencils.
@@ -53,8 +53,8 @@ __device__ __forceinline__ uint3 xcd_block()
 // cells to threads HP0..; each such thread keeps its ring cell's column in
 // registers the way it keeps its own.
 // Iteration k: psi(k+2) from phi loaded one iteration earlier; s(k+1); the
-// output plane k: convective and diffusive terms (A4, A5), the sharpening
-// divergence and phi^{n+1} = phi + dt*rhs (A10) written to a second buffer;
+// output plane k: convective and diffusive terms, the sharpening
+// divergence and phi^{n+1} = phi + dt*rhs written to a second buffer;
 // then psi(k+2) and phi(k+1) into LDS, and one barrier.  u, v of plane k+1, w
 // of plane k+2 and phi of plane k+3 are loaded at the start of iteration k, a
 // whole iteration ahead of their use.  HBM traffic:
@@ -167,17 +167,17 @@ __device__ __forceinline__ void k_acdi_body(uint3 blk,
     const int hP = (hc.y + 2) * APX + (hc.x + 2), hS = (hc.y + 1) * ASX + (hc.x + 1);
 
     auto ld_phi = [&](unsigned off, int m) { return ld_off(phi + nn * (size_t)wrap(m, n), off); };
-    auto psi_of = [&](T f) {                  // A6
+    auto psi_of = [&](T f) {                  // psi = eps log(phi / (1 - phi))
         T val = f < (T)1.0 ? f : (T)1.0;
-        if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative (Q7)
+        if constexpr (sizeof(T) == 4) val = val > (T)0.0 ? val : (T)0.0;   // FP32 phi: no log of a negative
         return eps * tlog(tdiv(val + en, ((T)1.0 - val) + en), tabL);
     };
-    // A7 + A8 + a(psi) at a cell: g* the undivided gradient of psi, pc0 psi
-    // and f phi at the cell.  The factor 1/4 (1 - tanh^2(psi/(2 eps))) of A9
+    // Normal and a(psi) at a cell: g* the undivided gradient of psi, pc0 psi
+    // and f phi at the cell.  The factor 1/4 (1 - tanh^2(psi/(2 eps))) of the flux
     // comes from the cell's phi, no tanh: psi = eps log(A/B), A = val + enum,
     // B = (1 - val) + enum, so it is A B / (A + B)^2 exactly (and, like the
     // tanh form, goes to 0 with A as phi nears -enum).  A NaN psi (phi below
-    // -enum, Q7) stays NaN, as tanh(NaN) would.
+    // -enum) stays NaN, as tanh(NaN) would.
     auto sharp = [&](T gx, T gy, T gz, T pc0, T f, T &sx, T &sy, T &sz) {
         T normod = tdiv((T)1.0, tsqrt((gx * gx + gy * gy) + gz * gz) + tiny);
         T val = f < (T)1.0 ? f : (T)1.0;
@@ -384,8 +384,8 @@ __global__ void __launch_bounds__(ATHREADS, sizeof(T) == 4 ? 8 : 4) k_acdi(
 
 
 // ---------------------------------------------------------------------------
-// Momentum, main.f90:513-681: A11 convection, A12 viscous, A13 ABC forcing,
-// A14 CSF force (phi^{n+1}) interpolated to the faces, A15 AB2/Euler advance
+// Momentum, main.f90:513-681: convection, viscous term, ABC forcing,
+// CSF force (phi^{n+1}) interpolated to the faces, AB2/Euler advance
 // and rhs_o <- rhs, in one pass.
 //
 // A block owns an MTX x MTY column and marches kz planes in z.  LDS rings
@@ -481,7 +481,7 @@ __device__ __forceinline__ void k_momentum_body(uint3 blk,
     };
     auto load_u = [&](int m) { fetch_u(m); store_u(m); };
     auto load_phi = [&](int m) { fetch_phi(m); store_phi(m); };
-    auto force = [&](int m) {           // A14 first half on plane m
+    auto force = [&](int m) {           // CSF force at cell centres (first half) on plane m
         if constexpr (PHI) {
             const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
             const T sig6 = (T)((T)6.0 * c.sigma), eps = (T)c.eps, epsi = (T)c.epsi;
@@ -647,7 +647,7 @@ struct MomK {
     T dxi, ddxi, mu, rhoi, f1, f2, f3, dt, alpha, beta;
 };
 
-// A11-A15 at one point, main.f90:513-696 (as in k_momentum).  U0..W0: plane
+// The momentum point, main.f90:513-696 (as in k_momentum).  U0..W0: plane
 // k, Up..Wp: plane k+1, with row stride SX; uzm..wzm, uxpm (u at k-1, i+1)
 // and vypm (v at k-1, j+1) from plane k-1; FX/FY/FZ the force of plane k and
 // fzm its z part at plane k-1 (row stride FXS).
@@ -806,7 +806,7 @@ __device__ __forceinline__ void k_momentum_x2_body(uint3 blk,
     };
     auto load_u = [&](int m) { fetch_u(m); store_u(m); };
     auto load_phi = [&](int m) { fetch_phi(m); store_phi(m); };
-    auto force = [&](int m) {           // A14 first half on plane m (as in k_momentum)
+    auto force = [&](int m) {           // CSF force at cell centres (first half) on plane m (as in k_momentum)
         if constexpr (PHI) {
             const int ps = (m - k0 + 2) % 3, psm = (m - 1 - k0 + 2) % 3, psp = (m + 1 - k0 + 2) % 3;
             const int fs = (m - k0 + 1) & 1;
@@ -975,7 +975,7 @@ __device__ void fft_lds(double2 *lds, int nlines, int lstride, int pstride, cons
     }
 }
 
-// A16 + A17 + forward x pass: divergence of u*, REAL(4) rounding of the
+// Forward x pass with the Poisson right-hand side: divergence of u*, REAL(4) rounding of the
 // Poisson RHS, then one complex FFT per pair of x-lines (j, j+1) whose
 // spectrum is split into the two half spectra (main.f90:715-750).
 template <typename T>
@@ -1069,7 +1069,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_ypass(double2 *__restrict__
 
 
 // Column pass along z: forward FFT, division by -k^2 and N^3 with the mean
-// mode set to zero (A19, main.f90:765-798), inverse FFT.  Block = (kx tile, j).
+// mode set to zero (main.f90:765-798), inverse FFT.  Block = (kx tile, j).
 // INV_ONLY: the inverse z transform alone (no forward transform, no divide),
 // for the initial-condition generator.
 template <bool INV_ONLY = false>
@@ -1127,7 +1127,7 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_zpoisson(double2 *__restric
 
 
 // Inverse x pass: Hermitian extension of the half spectra of lines j and j+1,
-// one complex inverse FFT per pair, p(.,j) = Re, p(.,j+1) = Im (A20-A21).
+// one complex inverse FFT per pair, p(.,j) = Re, p(.,j+1) = Im.
 __device__ __forceinline__ void k_xinv_body(uint3 blk, const double2 *__restrict__ spec,
                                                           double *__restrict__ p, const double2 *__restrict__ tw,
                                                           const Scalars *__restrict__ sc, FftPlan pl, int ppb,
@@ -1651,8 +1651,8 @@ __global__ void __launch_bounds__(FFT_MAX_THREADS) k_xinv_r(const double2 *__res
 
 
 // ---------------------------------------------------------------------------
-// A22 projection (main.f90:873-887) with the per-x-line partial sums and
-// maxima behind A23/A25.  One 64-wide wavefront per x-line: lane l owns
+// Projection (main.f90:873-887) with the per-x-line partial sums and
+// maxima behind the mean removal and umax.  One 64-wide wavefront per x-line: lane l owns
 // i = l, l+64, ..., adds them in that order, then a xor-butterfly combines the
 // 64 lanes.  That fixed order is what the oracle's lane-64 sum restates.
 // ---------------------------------------------------------------------------

@@ -96,29 +96,29 @@ class LiteralBox:
             P, Pxp, Pxm = phi[c], xp(phi)[c], xm(phi)[c]
             Pyp, Pym = phi[:, Jp, J], phi[:, Jm, J]
             Pzp, Pzm = phi[:, J, Jp], phi[:, J, Jm]
-            # K1 main.f90:353-355
+            # main.f90:353-355
             rhsphi[c] = (- (xp(u)[c] * 0.5 * (Pxp + P) - u[c] * 0.5 * (P + Pxm)) * dxi
                          - (v[:, Jp, J] * 0.5 * (Pyp + P) - v[c] * 0.5 * (P + Pym)) * dxi
                          - (w[:, J, Jp] * 0.5 * (Pzp + P) - w[c] * 0.5 * (P + Pzm)) * dxi)
-            # K2 main.f90:362-377
+            # main.f90:362-377
             gamma = self.gamma_factor * self.gumax
             rhsphi[c] = rhsphi[c] + gamma * (eps * (Pxp - 2.0 * P + Pxm) * ddxi
                                              + eps * (Pyp - 2.0 * P + Pym) * ddxi
                                              + eps * (Pzp - 2.0 * P + Pzm) * ddxi)
-            # K3 main.f90:384-393 (whole padded array)
+            # main.f90:384-393 (whole padded array)
             val = np.minimum(phi, 1.0)
             psidi = eps * np.log((val + self.enum) / (1.0 - val + self.enum))
-            # K4 main.f90:397-414 + halos 418-429
+            # main.f90:397-414 + halos 418-429
             normx, normy, normz = np.zeros_like(phi), np.zeros_like(phi), np.zeros_like(phi)
             normx[c] = xp(psidi)[c] - xm(psidi)[c]
             normy[c] = psidi[:, Jp, J] - psidi[:, Jm, J]
             normz[c] = psidi[:, J, Jp] - psidi[:, J, Jm]
             for a in (normx, normy, normz):
                 self.halo(a)
-            # K5 main.f90:433-442
+            # main.f90:433-442
             normod = 1.0 / (np.sqrt(normx ** 2 + normy ** 2 + normz ** 2) + float(np.float32(1.0E-16)))
             normx, normy, normz = normx * normod, normy * normod, normz * normod
-            # K6 main.f90:463-465
+            # main.f90:463-465
             th = lambda q: np.tanh(0.5 * q * epsi)
             fxp = 0.25 * (1.0 - th(xp(psidi)[c]) ** 2) * xp(normx)[c]
             fxm = 0.25 * (1.0 - th(xm(psidi)[c]) ** 2) * xm(normx)[c]
@@ -128,11 +128,11 @@ class LiteralBox:
             fzm = 0.25 * (1.0 - th(psidi[:, J, Jm]) ** 2) * normz[:, J, Jm]
             rhsphi[c] = rhsphi[c] - gamma * ((fxp - fxm) * 0.5 * dxi + (fyp - fym) * 0.5 * dxi
                                              + (fzp - fzm) * 0.5 * dxi)
-            # K7 main.f90:476 + halos 484-485
+            # main.f90:476 + halos 484-485
             phi[c] = phi[c] + dt * rhsphi[c]
             self.halo(phi)
 
-        # K8 convection main.f90:527-549
+        # convection main.f90:527-549
         U, V, W = u, v, w
         u0, v0, w0 = U[c], V[c], W[c]
         h11 = (xp(U)[c] + u0) * (xp(U)[c] + u0) - (u0 + xm(U)[c]) * (u0 + xm(U)[c])
@@ -148,7 +148,7 @@ class LiteralBox:
         rhsu = -(hs[0] + hs[1] + hs[2])
         rhsv = -(hs[3] + hs[4] + hs[5])
         rhsw = -(hs[6] + hs[7] + hs[8])
-        # K9 viscous main.f90:567-578
+        # viscous main.f90:567-578
         mu = self.mu
         lap = lambda A: (mu * (xp(A)[c] - 2.0 * A[c] + xm(A)[c]) * ddxi,
                          mu * (A[:, Jp, J] - 2.0 * A[c] + A[:, Jm, J]) * ddxi,
@@ -159,7 +159,7 @@ class LiteralBox:
         rhsv = rhsv + (a1 + a2 + a3) * rhoi
         a1, a2, a3 = lap(W)
         rhsw = rhsw + (a1 + a2 + a3) * rhoi
-        # K10 ABC forcing main.f90:585-593 (interior plane j at y = x(j - halo_ext))
+        # ABC forcing main.f90:585-593 (interior plane j at y = x(j - halo_ext))
         x = self.x
         k0 = self.k0
         sx = np.sin(k0 * x)[:, None, None]
@@ -171,7 +171,7 @@ class LiteralBox:
         rhsu = rhsu + self.f3 * sz + self.f2 * cy
         rhsv = rhsv + self.f1 * sx + self.f3 * cz
         rhsw = rhsw + self.f2 * sy + self.f1 * cx
-        # K11 + K12 CSF main.f90:606-657
+        # CSF main.f90:606-657
         if self.phiflag:
             phi = self.phi
             eps, epsi = self.eps, self.epsi
@@ -188,7 +188,7 @@ class LiteralBox:
             rhsu = rhsu + 0.5 * (xm(fxst)[c] + fxst[c]) * rhoi
             rhsv = rhsv + 0.5 * (fyst[:, Jm, J] + fyst[c]) * rhoi
             rhsw = rhsw + 0.5 * (fzst[:, J, Jm] + fzst[c]) * rhoi
-        # K13 + K14 main.f90:666-681, halos 686-695
+        # main.f90:666-681, halos 686-695
         ustar, vstar, wstar = np.zeros_like(u), np.zeros_like(u), np.zeros_like(u)
         ustar[c] = u0 + dt * (self.alpha * rhsu - self.beta * self.rhsu_o[c])
         vstar[c] = v0 + dt * (self.alpha * rhsv - self.beta * self.rhsv_o[c])
@@ -197,15 +197,15 @@ class LiteralBox:
         self.rhsu_o[c], self.rhsv_o[c], self.rhsw_o[c] = rhsu, rhsv, rhsw
         for a in (ustar, vstar, wstar):
             self.halo(a)
-        # K15 divergence main.f90:723-725
+        # divergence main.f90:723-725
         cd = self.rho * dxi / dt
         rhsp = cd * (xp(ustar)[c] - ustar[c])
         rhsp = rhsp + cd * (vstar[:, Jp, J] - vstar[c])
         rhsp = rhsp + cd * (wstar[:, J, Jp] - wstar[c])
-        # K16 main.f90:738
+        # main.f90:738
         if self.fp32:
             rhsp = rhsp.astype(np.float32).astype(np.float64)
-        # A18-A21: Z2Z forward, divide, inverse, real part
+        # Z2Z forward, divide, inverse, real part
         psi = np.fft.fftn(rhsp.astype(np.complex128), norm="backward")
         kk = self.kx
         k2 = kk[:, None, None] ** 2 + kk[None, :, None] ** 2 + kk[None, None, :] ** 2
@@ -216,13 +216,13 @@ class LiteralBox:
         p = np.zeros_like(u)
         p[c] = pint
         self.halo(p)
-        # K19 projection main.f90:881-883
+        # projection main.f90:881-883
         u[c] = ustar[c] - dt / self.rho * (p[c] - xm(p)[c]) * dxi
         v[c] = vstar[c] - dt / self.rho * (p[c] - p[:, Jm, J]) * dxi
         w[c] = wstar[c] - dt / self.rho * (p[c] - p[:, J, Jm]) * dxi
         for a in (u, v, w):
             self.halo(a)
-        # K20 + K21 mean removal main.f90:890-919
+        # mean removal main.f90:890-919
         gum = np.sum(u[c]) / n / n / n
         gvm = np.sum(v[c]) / n / n / n
         gwm = np.sum(w[c]) / n / n / n

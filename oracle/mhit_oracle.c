@@ -9,25 +9,25 @@
  *   orc_derive ............ multi/readinput.f90:35-43      pinned: val t=0 closed form
  *   grid / ktab ........... multi/main.f90:149-161          pinned: val t=0, Poisson KAT
  *   orc_init_ic ........... main.f90:218-233, 263-278       pinned: val t=0 dissipation
- *   phase field A4-A10 .... main.f90:338-486                pinned: ACDI equilibrium of the tanh
+ *   phase field ........... main.f90:338-486                pinned: ACDI equilibrium of the tanh
  *                                                            profile (2nd-order residual), Fourier-
  *                                                            mode eigenvalue and 2nd-order staggered
- *                                                            convergence of A4, sum(phi) invariant,
+ *                                                            convergence of the convection, sum(phi) invariant,
  *                                                            numpy literal restatement
- *   momentum A11-A15 ...... main.f90:513-696                pinned: literal restatement, val
+ *   momentum .............. main.f90:513-696                pinned: literal restatement, val
  *                                                            series at t=0.2, closed forms
- *   ABC forcing A13 ....... main.f90:584-600                pinned: pointwise closed form from rest
- *   CSF force A14 ......... main.f90:604-658                pinned: zero chemical potential of the
+ *   ABC forcing ........... main.f90:584-600                pinned: pointwise closed form from rest
+ *   CSF force ............. main.f90:604-658                pinned: zero chemical potential of the
  *                                                            plane profile (2nd order), Laplace jump
  *                                                            2 sigma/R and the drop's symmetries
  *   (tests/oracle_pins.py; tests/test_oracle_mutants.py plants 19 mistakes in
  *    copies of this file and shows that each is caught)
- *   divergence A16-A17 .... main.f90:715-745                pinned: literal restatement
- *   Poisson A18-A21 ....... main.f90:749-850                pinned: numpy.fft, brute-force DFT,
+ *   divergence ............ main.f90:715-745                pinned: literal restatement
+ *   Poisson ............... main.f90:749-850                pinned: numpy.fft, brute-force DFT,
  *                                                            discrete-mode KAT
- *   projection A22 ........ main.f90:873-887                pinned: literal restatement,
+ *   projection ............ main.f90:873-887                pinned: literal restatement,
  *                                                            div-free with modified k
- *   mean / umax A23-A25 ... main.f90:889-946                pinned: constructed sums, literal
+ *   mean / umax ........... main.f90:889-946                pinned: constructed sums, literal
  *   dissipation ........... post/compute_diss/main.f90:61-119 + module.f90:5 (REAL(4) pi)
  *                                                            pinned: val/ series, t=0 rows
  *   orc_log / orc_tanh .... own implementations (see below) pinned: libm within 1 ulp (log), 3 ulp (tanh)
@@ -175,13 +175,13 @@ void orc_derive(const orc_params *p, orc_consts *c)
 {
     c->twopi = 8.0 * atan(1.0);
     c->lx = c->twopi;
-    c->dx = c->lx / (double)(p->nx - 1);          /* Q1: dx = lx/(nx-1)      */
+    c->dx = c->lx / (double)(p->nx - 1);          /* dx = lx/(nx-1)      */
     c->dxi = 1.0 / c->dx;
     c->ddxi = 1.0 / c->dx / c->dx;
     c->rhoi = 1.0 / p->rho;
     c->eps = p->epsr * c->dx;
     c->epsi = 1.0 / c->eps;
-    c->enum_ = (double)1.e-16f;                   /* REAL(4) literal, Q9     */
+    c->enum_ = (double)1.e-16f;                   /* REAL(4) literal          */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -451,7 +451,7 @@ orc_state *orc_create(const orc_params *p)
     s->n3 = (size_t)n * n * n;
     size_t n3 = s->n3;
     s->x = zalloc(n); s->ktab = zalloc(n); s->sinf = zalloc(n); s->cosf = zalloc(n);
-    /* main.f90:149-159: cumulative grid, integer wavenumbers (Q2) */
+    /* main.f90:149-159: cumulative grid, integer wavenumbers */
     s->x[0] = 0.0;
     for (int i = 1; i < n; i++) s->x[i] = s->x[i - 1] + s->c.dx;
     for (int i = 0; i < n; i++) {
@@ -556,7 +556,7 @@ void orc_get_field(const orc_state *s, int fieldn, double *dst)
 }
 
 /* main.f90:218-233 (TG) and 263-278 (one drop).  Interior plane j (0-based)
- * sits at y = x[j]: the global index convention jg = j - halo_ext (Q4). */
+ * sits at y = x[j]: the global index convention jg = j - halo_ext. */
 void orc_init_ic(orc_state *s)
 {
     int n = s->n;
@@ -595,7 +595,7 @@ static void poisson_solve(int n, const fftplan *pl, const double *ktab, int fp32
 {
     int nh = n / 2 + 1;
     size_t ldk = (size_t)n * nh;        /* stride between k planes, complex */
-    /* A17: cmplx(rhsp, 0.0) is COMPLEX(4): the RHS is rounded to REAL(4). */
+    /* cmplx(rhsp, 0.0) is COMPLEX(4): the RHS is rounded to REAL(4). */
     if (fp32) {
         size_t n3 = (size_t)n * n * n;
         #pragma omp parallel for schedule(static)
@@ -636,7 +636,7 @@ static void poisson_solve(int n, const fftplan *pl, const double *ktab, int fp32
                     base[2 * (size_t)j * nh + 1] = z[2 * j + 1];
                 }
             }
-        /* z pass: forward, divide by -k^2 and N^3 (A19), inverse */
+        /* z pass: forward, divide by -k^2 and N^3, inverse */
         double nnn = (double)((int64_t)n * (int64_t)n * (int64_t)n);
         #pragma omp for schedule(static)
         for (int j = 0; j < n; j++)
@@ -727,7 +727,7 @@ static void phase_field(orc_state *s, double gamma)
     const double *u = s->u, *v = s->v, *w = s->w;
     double *phi = s->phi, *rphi = s->rphi, *psi = s->psi;
 
-    /* A4 + A5: convective and diffusive terms (main.f90:341-381) */
+    /* convective and diffusive terms (main.f90:341-381) */
     #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
         int kp = (k + 1) % n, km = (k + n - 1) % n;
@@ -750,13 +750,13 @@ static void phase_field(orc_state *s, double gamma)
             }
         }
     }
-    /* A6: psi over the whole field (main.f90:384-393) */
+    /* psi over the whole field (main.f90:384-393) */
     #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < s->n3; q++) {
         double val = phi[q] < 1.0 ? phi[q] : 1.0;
         psi[q] = eps * orc_log((val + en) / ((1.0 - val) + en));
     }
-    /* A7 + A8: undivided central normals, normalised (main.f90:397-443);
+    /* undivided central normals, normalised (main.f90:397-443);
      * the sharpening flux a*n (a = 0.25*(1 - tanh^2(psi/(2 eps)))) is formed
      * per cell once and used by both neighbours (same value either way). */
     const double tiny = (double)1.0E-16f;
@@ -776,7 +776,7 @@ static void phase_field(orc_state *s, double gamma)
                     /* psi = eps log(A/B), A = val + enum, B = (1 - val) + enum, so
                      * 1/4 (1 - tanh^2(psi/(2 eps))) = A B / (A + B)^2 exactly; like the
                      * tanh form it goes to 0 with A (phi -> -enum).  A NaN psi (phi below
-                     * -enum, Q7) stays NaN, as tanh(NaN) does. */
+                     * -enum) stays NaN, as tanh(NaN) does. */
                     double f = phi[IDX(i, j, k, n)];
                     double val = f < 1.0 ? f : 1.0;
                     double A = val + en, B = (1.0 - val) + en, S = A + B;
@@ -793,7 +793,7 @@ static void phase_field(orc_state *s, double gamma)
             }
         }
     }
-    /* A9 + A10: sharpening flux and Euler update (main.f90:446-480) */
+    /* sharpening flux and Euler update (main.f90:446-480) */
     double *phin = s->fx;   /* scratch for phi^{n+1} */
     #pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) {
@@ -823,7 +823,7 @@ static void momentum(orc_state *s, double alpha, double beta)
     const double *u = s->u, *v = s->v, *w = s->w;
     int pf = s->p.phiflag;
 
-    /* A14 first half: CSF force at cell centres (main.f90:604-627), phi^{n+1} */
+    /* first half: CSF force at cell centres (main.f90:604-627), phi^{n+1} */
     if (pf) {
         const double *phi = s->phi;
         const double eps = c->eps, epsi = c->epsi, sig6 = 6.0 * s->p.sigma;
@@ -863,7 +863,7 @@ static void momentum(orc_state *s, double alpha, double beta)
 #define U(a, b, cc) u[IDX(a, b, cc, n)]
 #define V(a, b, cc) v[IDX(a, b, cc, n)]
 #define W(a, b, cc) w[IDX(a, b, cc, n)]
-                /* A11 convection, main.f90:527-549 */
+                /* convection, main.f90:527-549 */
                 double h11 = (U(ip, j, k) + U(i, j, k)) * (U(ip, j, k) + U(i, j, k)) - (U(i, j, k) + U(im, j, k)) * (U(i, j, k) + U(im, j, k));
                 double h12 = (U(i, jp, k) + U(i, j, k)) * (V(i, jp, k) + V(im, jp, k)) - (U(i, j, k) + U(i, jm, k)) * (V(i, j, k) + V(im, j, k));
                 double h13 = (U(i, j, kp) + U(i, j, k)) * (W(i, j, kp) + W(im, j, kp)) - (U(i, j, k) + U(i, j, km)) * (W(i, j, k) + W(im, j, k));
@@ -879,7 +879,7 @@ static void momentum(orc_state *s, double alpha, double beta)
                 double ru = -((h11 + h12) + h13);
                 double rv = -((h21 + h22) + h23);
                 double rw = -((h31 + h32) + h33);
-                /* A12 viscous, main.f90:567-578 */
+                /* viscous, main.f90:567-578 */
                 h11 = mu * ((U(ip, j, k) - 2.0 * U(i, j, k)) + U(im, j, k)) * ddxi;
                 h12 = mu * ((U(i, jp, k) - 2.0 * U(i, j, k)) + U(i, jm, k)) * ddxi;
                 h13 = mu * ((U(i, j, kp) - 2.0 * U(i, j, k)) + U(i, j, km)) * ddxi;
@@ -895,17 +895,17 @@ static void momentum(orc_state *s, double alpha, double beta)
 #undef U
 #undef V
 #undef W
-                /* A13 ABC forcing at node coordinates, main.f90:591-593 */
+                /* ABC forcing at node coordinates, main.f90:591-593 */
                 ru = (ru + f3 * s->sinf[k]) + f2 * s->cosf[j];
                 rv = (rv + f1 * s->sinf[i]) + f3 * s->cosf[k];
                 rw = (rw + f2 * s->sinf[j]) + f1 * s->cosf[i];
-                /* A14 second half: CSF interpolated to faces, main.f90:652-654 */
+                /* second half: CSF interpolated to faces, main.f90:652-654 */
                 if (pf) {
                     ru = ru + 0.5 * (s->fx[IDX(im, j, k, n)] + s->fx[IDX(i, j, k, n)]) * rhoi;
                     rv = rv + 0.5 * (s->fy[IDX(i, jm, k, n)] + s->fy[IDX(i, j, k, n)]) * rhoi;
                     rw = rw + 0.5 * (s->fz[IDX(i, j, km, n)] + s->fz[IDX(i, j, k, n)]) * rhoi;
                 }
-                /* A15 AB2 / Euler, main.f90:666-681 */
+                /* AB2 / Euler, main.f90:666-681 */
                 size_t id = IDX(i, j, k, n);
                 s->us[id] = u[id] + dt * (alpha * ru - beta * s->ruo[id]);
                 s->vs[id] = v[id] + dt * (alpha * rv - beta * s->rvo[id]);
@@ -940,10 +940,10 @@ int orc_step(orc_state *s, int nsteps, orc_diag *d)
     for (int st = 0; st < nsteps; st++) {
         double alpha = s->nsteps_since_init == 0 ? 1.0 : 1.5;
         double beta = s->nsteps_since_init == 0 ? 0.0 : 0.5;
-        double gamma = s->p.gamma_factor * s->gumax;     /* Q8: lagged */
+        double gamma = s->p.gamma_factor * s->gumax;     /* lagged */
         if (s->p.phiflag) phase_field(s, gamma);
         momentum(s, alpha, beta);
-        /* A16 divergence, main.f90:716-728 */
+        /* divergence, main.f90:716-728 */
         const double cdiv = rho * dxi / dt;
         #pragma omp parallel for schedule(static)
         for (int k = 0; k < n; k++) {
@@ -961,7 +961,7 @@ int orc_step(orc_state *s, int nsteps, orc_diag *d)
             }
         }
         poisson_solve(n, &s->plan, s->ktab, s->p.poisson_rhs_fp32, s->rhsp, s->spec, s->pr);
-        /* A22 projection, main.f90:873-887 */
+        /* projection, main.f90:873-887 */
         const double dtr = dt / rho;
         const double *p = s->pr;
         #pragma omp parallel for schedule(static)
@@ -978,7 +978,7 @@ int orc_step(orc_state *s, int nsteps, orc_diag *d)
                 }
             }
         }
-        /* A23 mean removal, main.f90:890-920 (Q12 at one rank) */
+        /* mean removal, main.f90:890-920 (at one rank) */
         double dn = (double)n;
         gu = orc_sum3(s->u, n) / dn / dn / dn;
         gv = orc_sum3(s->v, n) / dn / dn / dn;
@@ -989,7 +989,7 @@ int orc_step(orc_state *s, int nsteps, orc_diag *d)
             s->v[q] = s->v[q] - gv;
             s->w[q] = s->w[q] - gw;
         }
-        /* A25 signed umax and CFL, main.f90:937-946 */
+        /* signed umax and CFL, main.f90:937-946 */
         int nan = 0;
         double uc = max3(s->u, s->n3, &nan), vc = max3(s->v, s->n3, &nan), wc = max3(s->w, s->n3, &nan);
         double mx = uc > vc ? uc : vc;

@@ -10,12 +10,12 @@ them on the oracle; tests/test_oracle_mutants.py runs them on copies of the
 oracle with one planted mistake each and requires every mistake to be caught.
 
 Reference anchors (multi/main.f90):
-  341-359  A4 convection of phi (face velocities times face averages of phi)
-  362-381  A5 diffusion eps*lap(phi), times gamma
-  384-443  A6-A8 psi = eps log(phi/(1-phi)), undivided normals, normalisation
-  446-486  A9-A10 sharpening flux 1/4 (1 - tanh^2(psi/(2 eps))) n, Euler update
-  584-600  A13 ABC forcing at node coordinates
-  604-658  A14 CSF force 6 sigma mu grad(phi), mu = phi(1-phi)(1-2phi)/eps - eps lap(phi),
+  341-359  convection of phi (face velocities times face averages of phi)
+  362-381  diffusion eps*lap(phi), times gamma
+  384-443  psi = eps log(phi/(1-phi)), undivided normals, normalisation
+  446-486  sharpening flux 1/4 (1 - tanh^2(psi/(2 eps))) n, Euler update
+  584-600  ABC forcing at node coordinates
+  604-658  CSF force 6 sigma mu grad(phi), mu = phi(1-phi)(1-2phi)/eps - eps lap(phi),
            interpolated to the faces
 """
 import numpy as np
@@ -43,7 +43,7 @@ def _lap(f, dx):
 
 
 # ---------------------------------------------------------------------------
-# A5-A10: the hyperbolic-tangent profile is the ACDI equilibrium.
+# the hyperbolic-tangent profile is the ACDI equilibrium.
 # For phi = 1/2 (1 - tanh((r - R)/(2 eps))), psi = eps log(phi/(1-phi)) is the
 # signed distance (R - r), 1/4 (1 - tanh^2(psi/(2 eps))) = phi (1 - phi), the
 # unit normal of psi is -r_hat and eps grad(phi) = -phi(1-phi) r_hat: the
@@ -80,9 +80,9 @@ def check_sphere_equilibrium(lib=None, form="identity"):
 
 
 # ---------------------------------------------------------------------------
-# A4: convection of phi.  With a uniform velocity U along axis a and a
+# convection of phi.  With a uniform velocity U along axis a and a
 # discrete Fourier mode phi_i = c0 + c1 cos(theta i + s) along the same axis
-# (gamma = 0, so only A4 acts), the central face averages give exactly
+# (gamma = 0, so only the convection acts), the central face averages give exactly
 #   rhs_i = U c1 sin(theta) sin(theta i + s) / dx,
 # the eigenvalue of the centred difference on that mode.
 # ---------------------------------------------------------------------------
@@ -106,7 +106,7 @@ def check_convection_mode(lib=None, form="identity", n=32, m=3, U=0.7, dt=1e-2):
 
 
 # ---------------------------------------------------------------------------
-# A4 staggering: u(i) sits on the face i - 1/2 (main.f90:723-725, 881-883)
+# staggering: u(i) sits on the face i - 1/2 (main.f90:723-725, 881-883)
 # and phi at the centre i.  Sampling smooth periodic u and phi there, the
 # discrete convective term must approach -(u phi)' at x_i at second order
 # (error ratio ~4 per doubling of N); velocities read from the wrong faces
@@ -144,7 +144,7 @@ def check_convection_staggering(lib=None, form="identity"):
 
 
 # ---------------------------------------------------------------------------
-# A14: the chemical potential mu = phi(1-phi)(1-2phi)/eps - eps lap(phi)
+# the chemical potential mu = phi(1-phi)(1-2phi)/eps - eps lap(phi)
 # vanishes on the planar tanh profile (eps phi'' = phi(1-phi)(1-2phi)/eps
 # there), so the CSF cell force 6 sigma mu grad(phi) of a plane slab is pure
 # truncation error: against its natural size 6 sigma |phi(1-phi)(1-2phi)/eps|
@@ -181,7 +181,7 @@ def check_csf_plane(lib=None, form="identity"):
 
 
 # ---------------------------------------------------------------------------
-# A14 + A16-A22: Laplace's law.  A drop of radius R at rest, no sharpening
+# Laplace's law.  A drop of radius R at rest, no sharpening
 # (gamma = 0), exact projection (FD2 modified wavenumbers, no REAL(4)
 # rounding): after one step the pressure holds the CSF force, and its jump
 # across the interface is 2 sigma / R up to O(eps/R).  The drop is centred on
@@ -218,7 +218,7 @@ def check_laplace(lib=None, form="identity"):
 
 
 # ---------------------------------------------------------------------------
-# A13: ABC forcing, at the node coordinates x_i = (i-1) dx of each index
+# ABC forcing, at the node coordinates x_i = (i-1) dx of each index
 # (main.f90:591-593).  From rest (no convection, no viscous term, no CSF) the
 # first, Euler, step gives pointwise
 #   u* = dt (f3 sin(k0 z_k) + f2 cos(k0 y_j))

@@ -96,7 +96,7 @@ def test_derive_matches_oracle_bitwise(n):
     got = derive(p)
     ref = orc.consts(nx=n, rho=1.3, epsr=0.75)
     assert got == ref
-    assert got["dx"] == got["lx"] / (n - 1)                 # Q1
+    assert got["dx"] == got["lx"] / (n - 1)                 # dx = lx/(nx-1)
     assert got["enum_"] == 1.0000000168623835e-16            # REAL(4) 1.e-16
 
 

@@ -1,5 +1,5 @@
 """GPU tests of the C-ABI behaviour around the step: field round trips, the
-CFL / NaN stop (main.f90:945 and Q7), Euler after a restart (main.f90:323),
+CFL / NaN stop (main.f90:945, and a NaN), Euler after a restart (main.f90:323),
 the readwrite.f90 file layout and the Fortran driver."""
 import os
 import shutil
@@ -50,7 +50,7 @@ def test_cfl_stop_matches_oracle():
 
 
 def test_nan_stop_keeps_phi_parity():
-    """A NaN (Q7) halts mid-call; the fields are those of the last step that
+    """A NaN halts mid-call; the fields are those of the last step that
     ran, identical to the oracle's."""
     n = 16
     kw = dict(nx=n, dt=1e-3, mu=0.01, phiflag=1, sigma=0.5)

@@ -46,7 +46,7 @@ def test_c1_tgv_64_200_steps():
 
 @pytest.mark.parametrize("n,wavenumbers,fp32", [(32, 0, 1), (32, 1, 0), (24, 0, 0)])
 def test_forced_random_ns(n, wavenumbers, fp32):
-    """ABC forcing (A13), random velocity, both wavenumber tables, with and
+    """ABC forcing, random velocity, both wavenumber tables, with and
     without the REAL(4) Poisson RHS rounding."""
     kw = dict(nx=n, dt=2e-3, mu=0.005, f1=1.0, f2=1.0, f3=1.0, k0=2.0, phiflag=0,
               wavenumbers=wavenumbers, poisson_rhs_fp32=fp32, inflow=1)

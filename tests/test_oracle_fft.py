@@ -52,7 +52,7 @@ def _mode(n, mx, my, mz):
                                  (32, (0, 0, 3))])
 def test_poisson_discrete_mode_kat(n, m):
     """RHS = a discrete Fourier mode with integer wavenumber m: under the
-    reference's integer-k table (Q2) p = -rhs/|m|^2 exactly.  This is the
+    reference's integer-k table p = -rhs/|m|^2 exactly.  This is the
     commented-out KAT of main.f90:29, 739-741, redone with a true DFT mode."""
     rhs = np.asfortranarray(_mode(n, *m))
     p = orc.poisson(rhs, poisson_rhs_fp32=0)

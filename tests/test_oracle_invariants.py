@@ -47,7 +47,7 @@ def test_sum3_restatement(n):
 
 
 def test_phase_field_conserves_total_phi():
-    """A4, A5 and A9 are flux differences on a periodic grid: sum(phi) is
+    """Convection, diffusion and the sharpening flux are flux differences on a periodic grid: sum(phi) is
     conserved to round-off by the phi update."""
     n = 24
     u, v, w = random_velocity(n, seed=5, amp=0.5)
@@ -63,7 +63,7 @@ def test_phase_field_conserves_total_phi():
 
 
 def test_forcing_mean_closed_form():
-    """A13 + A23: from rest, one Euler step with only f2 gives
+    """Forcing and mean removal: from rest, one Euler step with only f2 gives
     u* = dt*f2*cos(k0*x(j)); with dx = 2pi/(N-1) the grid sum of cos(2 x_j)
     is 1 (x_{N-1} = 2pi repeats x_0), so the removed mean is dt*f2/N."""
     n, dt, f2 = 16, 1e-3, 0.7
@@ -78,7 +78,7 @@ def test_forcing_mean_closed_form():
 def test_modified_wavenumbers_make_velocity_divergence_free():
     """With FD2 modified wavenumbers and no REAL(4) rounding the projection
     is exact: the discrete divergence (main.f90:723-725) of u^{n+1} is at
-    round-off, while integer wavenumbers (Q2) leave a residual."""
+    round-off, while integer wavenumbers leave a residual."""
     n = 16
     u, v, w = random_velocity(n, seed=9)
     res = {}
@@ -111,7 +111,7 @@ def test_cfl_stop_and_signed_umax():
 
 
 def test_nan_stops_the_run():
-    """Q7: phi below -enum makes psi NaN; it reaches u through CSF and stops."""
+    """phi below -enum makes psi NaN; it reaches u through CSF and stops."""
     n = 16
     o = orc.Oracle(nx=n, dt=1e-3, mu=0.01, phiflag=1, sigma=0.5)
     o.init_ic()
@@ -151,7 +151,7 @@ def test_sharpening_identity_matches_literal_tanh(fp32_rhs):
     through the identity A B / (A + B)^2 (A = val + enum, B = 1 - val + enum);
     the oracle has both forms, which differ by rounding only.  Over 200 steps of a forced flow with drops:
     - with an FP64 Poisson right-hand side the fields stay within 1e-13;
-    - with the reference's REAL(4) right-hand side (Q3, main.f90:738) one
+    - with the reference's REAL(4) right-hand side (main.f90:738) one
       rounding difference can flip a float rounding of rhsp, which moves p by
       an FP32 ulp; u, v, w, phi then stay within 1e-10 (the FP64 tolerance
       of the target) and p within 1e-7."""

@@ -1,4 +1,4 @@
-"""The oracle's phase field (A4-A10), CSF force (A14) and ABC forcing (A13)
+"""The oracle's phase field, CSF force and ABC forcing
 against what the mathematics fixes (tests/oracle_pins.py): the ACDI
 equilibrium of the tanh profile, the centred-difference eigenvalue of a
 Fourier mode, second-order convergence of the staggered convective term, the
