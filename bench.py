@@ -177,7 +177,7 @@ def ensemble_leg(count=16, steps=(200, 800), inflight=2, single_box_gcell_steps_
     updates per second and the step latency percentiles."""
     import numpy as np
     from mhit36_amd import serve
-    reqs = serve.c5_stream(count=count, steps=steps)
+    reqs = serve.serving_stream(count=count, steps=steps)
     cells = serve.cell_steps(reqs)
     serve.pretune(reqs)                      # launch shapes cached before either timed run
     t_one, out_one = serve.run_one_at_a_time(reqs)
@@ -206,7 +206,7 @@ def small_box_leg(count=32, steps=(200, 800)):
     shared launches (gangs of 8)."""
     import numpy as np
     from mhit36_amd import serve
-    reqs = serve.c5_stream(count=count, steps=steps, sizes=(64,))
+    reqs = serve.serving_stream(count=count, steps=steps, sizes=(64,))
     cells = serve.cell_steps(reqs)
     serve.pretune(reqs)
     t_one, out_one = serve.run_one_at_a_time(reqs)

@@ -13,7 +13,7 @@ from .solver import Solver, default_params
 SIZES = (64, 96, 128, 192, 256)
 
 
-def c5_stream(count=32, seed=2024, rate_hz=40.0, steps=(200, 2000), sizes=SIZES, precision=1, cover=False):
+def serving_stream(count=32, seed=2024, rate_hz=40.0, steps=(200, 2000), sizes=SIZES, precision=1, cover=False):
     """Requests: arrival time (s), parameters, step count and initial phi.
     N from `sizes`, nu in [0.002, 0.02], half of the boxes with ACDI drops
     (8-64 drops, radius 0.15-0.3); velocity is the built-in Taylor-Green field,

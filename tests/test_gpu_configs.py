@@ -46,7 +46,7 @@ def assert_same(s, o, fields):
         assert np.array_equal(s.get_field(f), o.get_field(f)), f
 
 
-def test_c2_256_forced_hit():
+def test_256_forced_hit():
     n = 256
     u, v, w, dx, umax = hit(n)
     kw = dict(nx=n, dt=0.3 * dx / umax, mu=0.005, f1=1.0, f2=1.0, f3=1.0, phiflag=0, inflow=1)
@@ -68,7 +68,7 @@ def test_c2_256_forced_hit():
             assert abs(a.mean()) < 1e-13 * np.abs(a).max()      # mean flow removed (main.f90:889-920)
 
 
-def test_c3_256_acdi_fp64_vs_mixed():
+def test_256_acdi_fp64_vs_mixed():
     n = 256
     u, v, w, dx, umax = hit(n)
     centres = ic.drop_centres(64, 0.25, dx, seed=4321)
@@ -101,7 +101,7 @@ def test_c3_256_acdi_fp64_vs_mixed():
         assert abs(a["phi_sum"] - b["phi_sum"]) <= 1e-5 * b["phi_sum"]
 
 
-def test_c4b_512_acdi_benchmark_box():
+def test_512_acdi_benchmark_box():
     import bench
     n = 512
     u, v, w, phi, dt = bench.make_inputs(n, 1)
@@ -117,7 +117,7 @@ def test_c4b_512_acdi_benchmark_box():
         o.close()
 
 
-def test_c4a_1024_ns_resident():
+def test_1024_ns_resident():
     """1024^3 forced HIT: the turbulent start generated on the device (u_rms 1,
     E(k) ~ k^4 exp(-2 (k/4)^2)), dt = 0.06 dx (CFL about 0.3 at the field's
     maximum), 100 steps in FP64 and in mixed precision, each context inside
@@ -127,7 +127,7 @@ def test_c4a_1024_ns_resident():
     kw = dict(nx=n, dt=0.06 * dx, mu=0.001, f1=1.0, f2=1.0, f3=1.0, k0=2.0, phiflag=0, inflow=1)
     res = {}
     for prec in (0, 1):
-        print(f"C4a 1024^3 precision={prec}", flush=True)
+        print(f"1024^3 NS-only precision={prec}", flush=True)
         with Solver(**kw, precision=prec) as s:
             assert s.device_bytes() < 288 * 10 ** 9
             s.init_random_velocity(seed=1234, k_peak=4.0, urms=1.0)
@@ -141,7 +141,7 @@ def test_c4a_1024_ns_resident():
     assert abs(res[1][1]["ke"] - res[0][1]["ke"]) <= 1e-5 * res[0][1]["ke"]
 
 
-def test_c4b_512_mixed_against_fp64():
+def test_512_mixed_against_fp64():
     """The benchmark box in mixed precision against FP64 over 100 steps."""
     import bench
     n = 512
@@ -149,7 +149,7 @@ def test_c4b_512_mixed_against_fp64():
     kw = bench.params_for(n, 1, dt)
     res = {}
     for prec in (0, 1):
-        print(f"C4b 512^3 100 steps precision={prec}", flush=True)
+        print(f"512^3 100 steps precision={prec}", flush=True)
         with Solver(**kw, precision=prec) as s:
             load(s, {1: u, 2: v, 3: w, 5: phi})
             d = s.step(100)

@@ -26,7 +26,7 @@ def check_fields(s, o, fields=(1, 2, 3, 4)):
 
 @pytest.mark.parametrize("n", [8, 16, 32, 48, 64, 96])
 def test_tgv_ns_bitwise(n):
-    """C1 shape: Taylor-Green, NS only (main.f90:218-233, 329-946)."""
+    """Taylor-Green, NS only (main.f90:218-233, 329-946)."""
     kw = dict(nx=n, dt=1e-3, mu=1.0 / 1600, phiflag=0)
     s, o, ds, do = run_pair(kw, 5, chunks=[1, 4])
     check_fields(s, o)
@@ -35,8 +35,8 @@ def test_tgv_ns_bitwise(n):
     s.close()
 
 
-def test_c1_tgv_64_200_steps():
-    """C1: 64^3 TGV, nu=1/1600, dt=1e-3, Euler then AB2, 200 steps."""
+def test_tgv_64_200_steps():
+    """64^3 TGV, nu=1/1600, dt=1e-3, Euler then AB2, 200 steps."""
     kw = dict(nx=64, dt=1e-3, mu=1.0 / 1600, phiflag=0)
     s, o, ds, do = run_pair(kw, 200, chunks=[1, 99, 100])
     check_fields(s, o)

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _stream():
-    reqs = serve.c5_stream(count=12, seed=7, rate_hz=40.0, steps=(200, 2000), cover=True)
+    reqs = serve.serving_stream(count=12, seed=7, rate_hz=40.0, steps=(200, 2000), cover=True)
     assert {r["params"]["nx"] for r in reqs} == set(serve.SIZES)
     assert any(r["phi"] is not None for r in reqs) and any(r["phi"] is None for r in reqs)
     # the FP64 box: the first 64^3 box with drops, cut to 200 steps for the oracle
@@ -38,9 +38,9 @@ def _alone(r):
         return {f: s.get_field(f) for f in ((1, 2, 3, 4, 5) if r["params"]["phiflag"] else (1, 2, 3, 4))}
 
 
-def test_c5_stream_same_alone_and_in_mix_and_fp64_box_equals_oracle():
+def test_serving_stream_same_alone_and_in_mix_and_fp64_box_equals_oracle():
     reqs, q64 = _stream()
-    print("C5 stream:", [(r["params"]["nx"], r["nsteps"], r["params"]["phiflag"], r["params"]["precision"])
+    print("serving stream:", [(r["params"]["nx"], r["nsteps"], r["params"]["phiflag"], r["params"]["precision"])
                          for r in reqs], flush=True)
     wall, out, lat, res = serve.run_ensemble(reqs, follow_arrivals=True, fields=(1, 2, 3, 4, 5), inflight=2)
     for q, r in enumerate(reqs):
@@ -71,7 +71,7 @@ def test_c5_stream_same_alone_and_in_mix_and_fp64_box_equals_oracle():
         for f in (1, 2, 3, 5):
             s.set_field(f, out[qm][f])
         dmx = s.diagnostics()
-    print(f"C5 mixed box {qm} (N={r['params']['nx']}, {r['nsteps']} steps) against FP64:",
+    print(f"serving stream: mixed box {qm} (N={r['params']['nx']}, {r['nsteps']} steps) against FP64:",
           {k: abs(dmx[k] - d64[k]) / abs(d64[k]) for k in ("ke", "diss", "phi_sum")}, flush=True)
     assert abs(dmx["ke"] - d64["ke"]) <= 1e-4 * d64["ke"]
     assert abs(dmx["diss"] - d64["diss"]) <= 1e-4 * d64["diss"]

@@ -8,7 +8,7 @@ from mhit36_amd import serve
 
 
 def stream(seed, count=12):
-    return serve.c5_stream(count=count, seed=seed, steps=(200, 2000), sizes=(64,))
+    return serve.serving_stream(count=count, seed=seed, steps=(200, 2000), sizes=(64,))
 
 
 def test_reproducible():
@@ -18,7 +18,7 @@ def test_reproducible():
         assert (x["phi"] is None) == (y["phi"] is None)
         if x["phi"] is not None:
             assert np.array_equal(x["phi"], y["phi"])
-    assert [r["params"] for r in serve.c5_stream(count=3, seed=6, sizes=(64,))] != [r["params"] for r in a[:3]]
+    assert [r["params"] for r in serve.serving_stream(count=3, seed=6, sizes=(64,))] != [r["params"] for r in a[:3]]
 
 
 def test_ranges():
@@ -40,5 +40,5 @@ def test_ranges():
 
 
 def test_cell_steps():
-    reqs = serve.c5_stream(count=4, seed=3, sizes=(64,))
+    reqs = serve.serving_stream(count=4, seed=3, sizes=(64,))
     assert serve.cell_steps(reqs) == sum(r["params"]["nx"] ** 3 * r["nsteps"] for r in reqs)
